@@ -1,0 +1,237 @@
+/*
+ * nli.h -- C-ABI of libnli, the MI355X (gfx950) implementation of the inner
+ * inpainting loop of Tom-Zheng/non-local-inpainting: PatchMatch NNF search and
+ * the non-local image updates (means, medians, Poisson with its conjugate
+ * gradient, depth), plus the pyramid, mask and confidence steps that surround
+ * them and the whole multiscale process().
+ *
+ * Conventions (the reference containers, image.hpp:481-485, mask.h):
+ *   images  float32 interleaved HWC, index C*(W*y + x) + c
+ *   masks   uint8 0/1, index W*y + x (1 = masked / hole / in the set)
+ *   NNF     int32 pairs (x, y), index 2*(W*y + x); (-1, -1) = no entry
+ * Unless a name ends in _dev, pointer arguments are HOST pointers: the call
+ * copies its inputs to the device, computes, and copies its outputs back before
+ * returning (the per-call semantics of the reference's host API).  _dev entry
+ * points take device pointers and run on the context's stream.
+ *
+ * Every function returns NLI_OK (0) or a negative NLI_E_* code and never throws;
+ * nli_last_error() gives the message.  Functions that fail where the reference
+ * throws std::runtime_error report NLI_E_ARGUMENT with the reference's message.
+ * A context is used by one host thread at a time.
+ */
+#ifndef NLI_H_
+#define NLI_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NLI_OK 0
+#define NLI_E_ARGUMENT (-1)     /* invalid argument; reference std::runtime_error cases */
+#define NLI_E_DEVICE (-2)       /* HIP runtime error */
+#define NLI_E_EMPTY_SOURCE (-3) /* no complete source patch (image_inpainting.cpp:382-384) */
+#define NLI_E_INTERNAL (-4)
+
+typedef struct nli_ctx nli_ctx;
+
+/* patch distance kinds (main.cpp:245-263 pairs them with the updaters) */
+#define NLI_DIST_L2 0          /* L2NormPatchDistance      l2_norm_patch_distance.h     */
+#define NLI_DIST_L1 1          /* L1NormPatchDistance      l1_norm_patch_distance.h     */
+#define NLI_DIST_L2COMBINED 2  /* L2CombinedPatchDistance  l2_combined_patch_distance.h */
+#define NLI_DIST_L2RGBD 3      /* L2RGBDPatchDistance      l2_rgbd_patch_distance.h     */
+
+/* image updating kinds */
+#define NLI_UPD_MEANS 0        /* PatchNonLocalMeans    patch_non_local_means.h   */
+#define NLI_UPD_MEDIANS 1      /* PatchNonLocalMedians  patch_non_local_medians.h */
+#define NLI_UPD_POISSON 2      /* PatchNonLocalPoisson  patch_non_local_poisson.h */
+#define NLI_UPD_DEPTH 3        /* PatchNonLocalDepth    patch_non_local_depth.h   */
+
+/* coarsest-level initialisation, ImageInpainting::InitType order (image_inpainting.h:43) */
+#define NLI_INIT_BLACK 0
+#define NLI_INIT_AVG 1
+#define NLI_INIT_NONE 2
+#define NLI_INIT_POISSON 3
+
+/* CG arithmetic */
+#define NLI_CG_FP64 0          /* the reference recurrence in double precision */
+#define NLI_CG_MIXED 1         /* FP32 vectors, FP64 dot products, periodic FP64 residual */
+
+typedef struct {
+    int kind;        /* NLI_DIST_* */
+    int patch;       /* square patch side, 1..15 */
+    float sigma;     /* Gaussian sigma of the patch weights */
+    float lambda;    /* L2Combined lambda / L2RGBD lambda_rgb */
+    float lambda_d;  /* L2RGBD lambda_d */
+} nli_dist_params;
+
+typedef struct {
+    int iterations;  /* sweeps (PatchMatch iteration_count, patch_match.h:41-47) */
+    int shots;       /* random shots limit */
+    int window;      /* search window size, -1 = max(W, H) */
+    int first_step;  /* jump-flood step of the first sweep, 0 = automatic */
+} nli_pm_params;
+
+typedef struct {
+    int kind;        /* NLI_UPD_* */
+    int patch;
+    float sigma;
+    float lambda;    /* Poisson lambda / Depth lambda_rgb */
+    float lambda_d;  /* Depth lambda_d */
+    float cg_tol;    /* Poisson: stop when |r|^2 < cg_tol |r0|^2 (Depth uses 1e-6) */
+    int cg_limit;    /* Poisson: iteration limit (Depth uses 1000) */
+    int cg_precision;      /* NLI_CG_FP64 or NLI_CG_MIXED */
+    int cg_refine_every;   /* mixed: FP64 residual replacement period (0 = never) */
+} nli_upd_params;
+
+typedef struct {
+    int channels;            /* channels solved (3 RGB + 1 depth for Depth) */
+    int cg_iterations[4];
+    double cg_nr0[4];        /* |r0|^2 per channel */
+    double cg_nr[4];         /* final |r|^2 per channel */
+} nli_cg_stats;
+
+typedef struct {
+    int method;      /* NLI_UPD_*: selects the (distance, updater) pair as main.cpp:245-263 */
+    int patch;
+    float sigma;
+    float lambda;
+    float lambda_rgb;
+    float lambda_d;
+    int iters;       /* inpainting iterations per scale */
+    float tol;       /* stop a scale when the total squared update <= tol */
+    int scales;
+    float coarse;    /* coarsest size ratio; the subsampling rate is exp(log(coarse)/(scales-1)) */
+    float conf_t;    /* confidence decay time */
+    float conf_a;    /* confidence asymptotic value */
+    int init;        /* NLI_INIT_* */
+    int pm_iters;
+    int pm_shots;
+    int pm_window;
+    float cg_tol;
+    int cg_limit;
+    int cg_precision;
+    int cg_refine_every;
+    uint64_t seed;
+} nli_inpaint_params;
+
+typedef struct {
+    int scales;
+    int iterations[32];      /* inpainting iterations run per scale, finest = 0 */
+    int width[32], height[32];
+    int hole_pixels[32];
+    int target_pixels[32];
+    int cg_iterations_total;
+    double seconds_total;
+    double seconds_patchmatch;
+    double seconds_update;
+    double seconds_other;
+} nli_inpaint_stats;
+
+/* ---- context ---- */
+int nli_ctx_create(int device, nli_ctx **out);
+int nli_ctx_destroy(nli_ctx *ctx);
+const char *nli_last_error(const nli_ctx *ctx);
+int nli_synchronize(nli_ctx *ctx);
+int nli_version(void);
+
+/* ---- device memory helpers for the _dev entry points ---- */
+int nli_dev_alloc(nli_ctx *ctx, uint64_t bytes, void **out);
+int nli_dev_free(nli_ctx *ctx, void *p);
+int nli_copy_h2d(nli_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int nli_copy_d2h(nli_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* ---- numerics ---- */
+/* replaces Gradient::calculate (gradient.h:25, gradient.cpp:20-76): out W*H*2C */
+int nli_gradient(nli_ctx *ctx, const float *img, int W, int H, int C, float *out);
+
+/* replaces DistanceTransform::calculate (distance_transform.h:30) */
+int nli_distance_transform(nli_ctx *ctx, const uint8_t *mask, int W, int H, float *out);
+
+/* replaces ImageInpainting::calculate_confidence_mask (image_inpainting.cpp:611-630) */
+int nli_confidence(nli_ctx *ctx, const uint8_t *mask, int W, int H, float decay, float asym,
+                   float *out);
+
+/* replaces Sampling::_Details::get_sample_size (sampling.cpp:408-411) and
+ * ImageInpainting::calculate_subsampling_rate (image_inpainting.cpp:197-204) */
+int nli_sample_size(int size, float factor);
+float nli_subsampling_rate(float ratio, int scales);
+
+/* replaces Sampling::downsample_with_mask (sampling.h:46, sampling.cpp:114-202);
+ * out_img sW*sH*C, out_mask sW*sH with sW = nli_sample_size(W, factor) */
+int nli_downsample_with_mask(nli_ctx *ctx, const float *img, const uint8_t *mask, int W, int H,
+                             int C, float factor, float threshold, int mask_val, float *out_img,
+                             uint8_t *out_mask);
+/* replaces Sampling::downsample(image) and Sampling::downsample(mask) (sampling.h:38-44) */
+int nli_downsample(nli_ctx *ctx, const float *img, int W, int H, int C, float factor, float *out);
+int nli_downsample_mask(nli_ctx *ctx, const uint8_t *mask, int W, int H, float factor,
+                        float threshold, uint8_t *out);
+/* replaces Sampling::upsample (sampling.h:52, sampling.cpp:205-246) */
+int nli_upsample(nli_ctx *ctx, const float *img, int W, int H, int C, int W2, int H2, float *out);
+
+/* ---- per-scale masks ---- */
+/* replaces ImageInpainting::get_extended_domain (image_inpainting.cpp:566-600) */
+int nli_extended_domain(nli_ctx *ctx, const uint8_t *hole, int W, int H, int patch, uint8_t *out);
+/* the mask preparation of inpaint_internal (image_inpainting.cpp:359-384, erode = 1) and of
+ * propagate_weights (:496-502, :543-548, erode = 0): extended domain, target, source.
+ * Returns NLI_E_EMPTY_SOURCE when erode = 1 and no source pixel survives the erosion. */
+int nli_scale_masks(nli_ctx *ctx, const uint8_t *hole, int W, int H, int patch, int erode,
+                    uint8_t *ext, uint8_t *target, uint8_t *source);
+
+/* ---- patch distances and PatchMatch ---- */
+/* APatchDistance::calculate (a_patch_distance.h:32-35) for every target point of a fixed
+ * field: out[W*y+x] = distance(nnf(x,y), (x,y)) on target points, FLT_MAX elsewhere. */
+int nli_patch_distance(nli_ctx *ctx, const nli_dist_params *dist, const float *src, int sW,
+                       int sH, const float *tgt, int tW, int tH, int C, const int32_t *nnf,
+                       const uint8_t *tmask, float *out);
+
+/* replaces PatchMatch::calculate (patch_match.h:50-54, patch_match.cpp:57-512).
+ * init_nnf = NULL is the reference's empty initial field (random initialisation).
+ * out_dist (optional) receives each target's distance (FLT_MAX elsewhere).
+ * Returns NLI_E_ARGUMENT on a size mismatch (the reference returns an empty field). */
+int nli_patchmatch(nli_ctx *ctx, const nli_dist_params *dist, const nli_pm_params *pm,
+                   const float *src, const uint8_t *smask, int sW, int sH, const float *tgt,
+                   const uint8_t *tmask, int tW, int tH, int C, const int32_t *init_nnf,
+                   uint64_t seed, int32_t *out_nnf, float *out_dist);
+int nli_patchmatch_dev(nli_ctx *ctx, const nli_dist_params *dist, const nli_pm_params *pm,
+                       const float *src, const uint8_t *smask, int sW, int sH, const float *tgt,
+                       const uint8_t *tmask, int tW, int tH, int C, const int32_t *init_nnf,
+                       uint64_t seed, int32_t *out_nnf, float *out_dist);
+
+/* ---- image updating ---- */
+/* replaces AImageUpdating::update (a_image_updating.h:34-39) of the four updaters:
+ * image (W*H*C) is updated in place on the hole pixels (domain); orig is the image the
+ * scale started from; ext the extended domain; conf may be NULL (empty confidence).
+ * *total_diff receives the returned value of the reference (sum of squared changes). */
+int nli_update(nli_ctx *ctx, const nli_upd_params *upd, float *image, const float *orig,
+               const uint8_t *domain, const uint8_t *ext, const int32_t *nnf, const float *conf,
+               int W, int H, int C, double *total_diff, nli_cg_stats *stats);
+int nli_update_dev(nli_ctx *ctx, const nli_upd_params *upd, float *image, const float *orig,
+                   const uint8_t *domain, const uint8_t *ext, const int32_t *nnf,
+                   const float *conf, int W, int H, int C, double *total_diff,
+                   nli_cg_stats *stats);
+
+/* ---- coarsest-level initialisation ---- */
+/* replaces poisson_solver_separable_simplest (3rdparty/simpois/simpois.c:281-291) as
+ * called by ImageInpainting::process (image_inpainting.cpp:125-153): fills the masked
+ * pixels of img (W*H*C, HWC) in place. Host computation (coarsest level only). */
+int nli_poisson_fill(float *img, const uint8_t *mask, int W, int H, int C);
+
+/* ---- the whole multiscale inpainting ---- */
+/* replaces ImageInpainting::process (image_inpainting.h:54, image_inpainting.cpp:68-190)
+ * with the plugins of main.cpp:245-266; runs device-resident. out: W*H*C. */
+int nli_inpaint(nli_ctx *ctx, const nli_inpaint_params *params, const float *in,
+                const uint8_t *mask, int W, int H, int C, float *out, nli_inpaint_stats *stats);
+int nli_inpaint_dev(nli_ctx *ctx, const nli_inpaint_params *params, const float *in,
+                    const uint8_t *mask, int W, int H, int C, float *out,
+                    nli_inpaint_stats *stats);
+
+/* fills params with the reference CLI defaults (main.cpp:68-82, 220-233, 254, 266) */
+void nli_inpaint_params_default(nli_inpaint_params *p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NLI_H_ */

@@ -1,0 +1,382 @@
+"""Python binding of libnli, the MI355X (gfx950) implementation of the inner
+inpainting loop of Tom-Zheng/non-local-inpainting.
+
+All computation happens in ``libnli.so`` (hand-written HIP kernels behind the
+C-ABI of ``include/nli.h``); this module only moves numpy arrays across that
+boundary with ctypes.  There is no CPU fallback: if the library is missing or
+no GPU is visible, the calls raise.
+
+Arrays follow the reference containers: images float32 (H, W, C) HWC, masks
+uint8 (H, W) with 1 = in the set, NNF int32 (H, W, 2) holding (x, y) with
+(-1, -1) = no entry.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnli.so")
+
+NLI_OK = 0
+NLI_E_ARGUMENT = -1
+NLI_E_DEVICE = -2
+NLI_E_EMPTY_SOURCE = -3
+NLI_E_INTERNAL = -4
+
+DIST = {"l2": 0, "l1": 1, "l2combined": 2, "l2rgbd": 3}
+UPD = {"nlmeans": 0, "nlmedians": 1, "nlpoisson": 2, "nldepth": 3}
+INIT = {"black": 0, "avg": 1, "none": 2, "poisson": 3}
+CG_FP64, CG_MIXED = 0, 1
+
+# every symbol include/nli.h declares
+EXPORTS = [
+    "nli_ctx_create", "nli_ctx_destroy", "nli_last_error", "nli_synchronize", "nli_version",
+    "nli_dev_alloc", "nli_dev_free", "nli_copy_h2d", "nli_copy_d2h",
+    "nli_gradient", "nli_distance_transform", "nli_confidence", "nli_sample_size",
+    "nli_subsampling_rate", "nli_downsample_with_mask", "nli_downsample", "nli_downsample_mask",
+    "nli_upsample", "nli_extended_domain", "nli_scale_masks", "nli_patch_distance",
+    "nli_patchmatch", "nli_patchmatch_dev", "nli_update", "nli_update_dev", "nli_poisson_fill",
+    "nli_inpaint", "nli_inpaint_dev", "nli_inpaint_params_default",
+]
+
+
+class NliError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{message} (code {code})")
+        self.code = code
+
+
+class EmptySourceError(NliError):
+    pass
+
+
+class DistParams(ct.Structure):
+    _fields_ = [("kind", ct.c_int), ("patch", ct.c_int), ("sigma", ct.c_float),
+                ("lambda_", ct.c_float), ("lambda_d", ct.c_float)]
+
+
+class PMParams(ct.Structure):
+    _fields_ = [("iterations", ct.c_int), ("shots", ct.c_int), ("window", ct.c_int),
+                ("first_step", ct.c_int)]
+
+
+class UpdParams(ct.Structure):
+    _fields_ = [("kind", ct.c_int), ("patch", ct.c_int), ("sigma", ct.c_float),
+                ("lambda_", ct.c_float), ("lambda_d", ct.c_float), ("cg_tol", ct.c_float),
+                ("cg_limit", ct.c_int), ("cg_precision", ct.c_int), ("cg_refine_every", ct.c_int)]
+
+
+class CGStats(ct.Structure):
+    _fields_ = [("channels", ct.c_int), ("cg_iterations", ct.c_int * 4),
+                ("cg_nr0", ct.c_double * 4), ("cg_nr", ct.c_double * 4)]
+
+
+class InpaintParams(ct.Structure):
+    _fields_ = [("method", ct.c_int), ("patch", ct.c_int), ("sigma", ct.c_float),
+                ("lambda_", ct.c_float), ("lambda_rgb", ct.c_float), ("lambda_d", ct.c_float),
+                ("iters", ct.c_int), ("tol", ct.c_float), ("scales", ct.c_int),
+                ("coarse", ct.c_float), ("conf_t", ct.c_float), ("conf_a", ct.c_float),
+                ("init", ct.c_int), ("pm_iters", ct.c_int), ("pm_shots", ct.c_int),
+                ("pm_window", ct.c_int), ("cg_tol", ct.c_float), ("cg_limit", ct.c_int),
+                ("cg_precision", ct.c_int), ("cg_refine_every", ct.c_int), ("seed", ct.c_uint64)]
+
+
+class InpaintStats(ct.Structure):
+    _fields_ = [("scales", ct.c_int), ("iterations", ct.c_int * 32), ("width", ct.c_int * 32),
+                ("height", ct.c_int * 32), ("hole_pixels", ct.c_int * 32),
+                ("target_pixels", ct.c_int * 32), ("cg_iterations_total", ct.c_int),
+                ("seconds_total", ct.c_double), ("seconds_patchmatch", ct.c_double),
+                ("seconds_update", ct.c_double), ("seconds_other", ct.c_double)]
+
+
+_f32p = ct.POINTER(ct.c_float)
+_u8p = ct.POINTER(ct.c_uint8)
+_i32p = ct.POINTER(ct.c_int32)
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> ct.CDLL:
+    """Load libnli.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run __graft_entry__.build() "
+                          "(make -C non-local-inpainting_amd)")
+    lib = ct.CDLL(path)
+    lib.nli_last_error.restype = ct.c_char_p
+    lib.nli_subsampling_rate.restype = ct.c_float
+    lib.nli_subsampling_rate.argtypes = [ct.c_float, ct.c_int]
+    lib.nli_sample_size.argtypes = [ct.c_int, ct.c_float]
+    lib.nli_ctx_create.argtypes = [ct.c_int, ct.POINTER(ct.c_void_p)]
+    lib.nli_ctx_destroy.argtypes = [ct.c_void_p]
+    lib.nli_last_error.argtypes = [ct.c_void_p]
+    _lib = lib
+    return lib
+
+
+def _ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+@dataclass
+class Inpainting:
+    """Parameters of ImageInpainting::process with its method pairing
+    (main.cpp:245-266).  Defaults are the reference CLI's, patch side 9."""
+    method: str = "nlmeans"
+    patch: int = 9
+    sigma: float = 10000.0
+    lam: float = 0.05
+    lambda_rgb: float = 0.05
+    lambda_d: float = 0.05
+    iters: int = 200
+    tol: float = 0.1
+    scales: int = 11
+    coarse: float = 0.1
+    conf_t: float = 5.0
+    conf_a: float = 0.1
+    init: str = "poisson"
+    pm_iters: int = 10
+    pm_shots: int = 30
+    pm_window: int = -1
+    cg_tol: float = 1e-6
+    cg_limit: int = 1000
+    cg_precision: int = CG_FP64
+    cg_refine_every: int = 0
+    seed: int = 1
+
+    def to_struct(self) -> InpaintParams:
+        return InpaintParams(UPD[self.method], self.patch, self.sigma, self.lam, self.lambda_rgb,
+                             self.lambda_d, self.iters, self.tol, self.scales, self.coarse,
+                             self.conf_t, self.conf_a, INIT[self.init], self.pm_iters, self.pm_shots,
+                             self.pm_window, self.cg_tol, self.cg_limit, self.cg_precision,
+                             self.cg_refine_every, self.seed)
+
+
+@dataclass
+class RunStats:
+    scales: int
+    iterations: list
+    sizes: list
+    hole_pixels: list
+    target_pixels: list
+    cg_iterations_total: int
+    seconds_total: float
+    seconds_patchmatch: float
+    seconds_update: float
+    extra: dict = field(default_factory=dict)
+
+
+class Context:
+    """One device context (stream + device buffers) of libnli."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = ct.c_void_p()
+        rc = self.lib.nli_ctx_create(device, ct.byref(h))
+        if rc != NLI_OK:
+            raise NliError(rc, f"nli_ctx_create(device={device}) failed: no usable GPU")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nli_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc == NLI_OK:
+            return
+        msg = self.lib.nli_last_error(self.h).decode(errors="replace")
+        if rc == NLI_E_EMPTY_SOURCE:
+            raise EmptySourceError(rc, msg)
+        if rc == NLI_E_ARGUMENT:
+            raise ValueError(msg)
+        raise NliError(rc, msg)
+
+    # ---- numerics ----
+    def gradient(self, img):
+        img = _c(img, np.float32)
+        H, W, C = img.shape
+        out = np.zeros((H, W, 2 * C), np.float32)
+        self._check(self.lib.nli_gradient(self.h, _ptr(img, _f32p), W, H, C, _ptr(out, _f32p)))
+        return out
+
+    def distance_transform(self, mask):
+        mask = _c(mask, np.uint8)
+        H, W = mask.shape
+        out = np.zeros((H, W), np.float32)
+        self._check(self.lib.nli_distance_transform(self.h, _ptr(mask, _u8p), W, H, _ptr(out, _f32p)))
+        return out
+
+    def confidence(self, mask, decay=5.0, asym=0.1):
+        mask = _c(mask, np.uint8)
+        H, W = mask.shape
+        out = np.zeros((H, W), np.float32)
+        self._check(self.lib.nli_confidence(self.h, _ptr(mask, _u8p), W, H, ct.c_float(decay),
+                                            ct.c_float(asym), _ptr(out, _f32p)))
+        return out
+
+    def sample_size(self, size, factor):
+        return int(self.lib.nli_sample_size(size, factor))
+
+    def downsample_with_mask(self, img, mask, factor, threshold=0.4, mask_val=False):
+        img = _c(img, np.float32)
+        mask = _c(mask, np.uint8)
+        H, W, C = img.shape
+        sw, sh = self.sample_size(W, factor), self.sample_size(H, factor)
+        oi = np.zeros((sh, sw, C), np.float32)
+        om = np.zeros((sh, sw), np.uint8)
+        self._check(self.lib.nli_downsample_with_mask(self.h, _ptr(img, _f32p), _ptr(mask, _u8p), W, H, C,
+                                                      ct.c_float(factor), ct.c_float(threshold), int(mask_val),
+                                                      _ptr(oi, _f32p), _ptr(om, _u8p)))
+        return oi, om
+
+    def downsample(self, img, factor):
+        img = _c(img, np.float32)
+        H, W, C = img.shape
+        sw, sh = self.sample_size(W, factor), self.sample_size(H, factor)
+        out = np.zeros((sh, sw, C), np.float32)
+        self._check(self.lib.nli_downsample(self.h, _ptr(img, _f32p), W, H, C, ct.c_float(factor),
+                                            _ptr(out, _f32p)))
+        return out
+
+    def downsample_mask(self, mask, factor, threshold=0.4):
+        mask = _c(mask, np.uint8)
+        H, W = mask.shape
+        sw, sh = self.sample_size(W, factor), self.sample_size(H, factor)
+        out = np.zeros((sh, sw), np.uint8)
+        self._check(self.lib.nli_downsample_mask(self.h, _ptr(mask, _u8p), W, H, ct.c_float(factor),
+                                                 ct.c_float(threshold), _ptr(out, _u8p)))
+        return out
+
+    def upsample(self, img, W2, H2):
+        img = _c(img, np.float32)
+        H, W, C = img.shape
+        out = np.zeros((H2, W2, C), np.float32)
+        self._check(self.lib.nli_upsample(self.h, _ptr(img, _f32p), W, H, C, W2, H2, _ptr(out, _f32p)))
+        return out
+
+    # ---- masks ----
+    def extended_domain(self, hole, patch):
+        hole = _c(hole, np.uint8)
+        H, W = hole.shape
+        out = np.zeros((H, W), np.uint8)
+        self._check(self.lib.nli_extended_domain(self.h, _ptr(hole, _u8p), W, H, patch, _ptr(out, _u8p)))
+        return out
+
+    def scale_masks(self, hole, patch, erode=True):
+        """(extended, target, source) of one scale; raises EmptySourceError when the
+        eroded source is empty (erode=True only)."""
+        hole = _c(hole, np.uint8)
+        H, W = hole.shape
+        e, t, s = (np.zeros((H, W), np.uint8) for _ in range(3))
+        self._check(self.lib.nli_scale_masks(self.h, _ptr(hole, _u8p), W, H, patch, int(erode), _ptr(e, _u8p),
+                                             _ptr(t, _u8p), _ptr(s, _u8p)))
+        return e, t, s
+
+    # ---- distances / PatchMatch ----
+    def patch_distance(self, kind, src, tgt, nnf, tmask, patch=9, sigma=10000.0, lam=0.5, lam_d=0.5):
+        src = _c(src, np.float32)
+        tgt = _c(tgt, np.float32)
+        nnf = _c(nnf, np.int32)
+        tmask = _c(tmask, np.uint8)
+        sH, sW, C = src.shape
+        tH, tW, _ = tgt.shape
+        dp = DistParams(DIST[kind], patch, sigma, lam, lam_d)
+        out = np.zeros((tH, tW), np.float32)
+        self._check(self.lib.nli_patch_distance(self.h, ct.byref(dp), _ptr(src, _f32p), sW, sH, _ptr(tgt, _f32p),
+                                                tW, tH, C, _ptr(nnf, _i32p), _ptr(tmask, _u8p), _ptr(out, _f32p)))
+        return out
+
+    def patchmatch(self, kind, src, smask, tgt, tmask, init=None, seed=1, patch=9, sigma=10000.0, lam=0.5,
+                   lam_d=0.5, iters=10, shots=30, window=-1, first_step=0, with_dist=False):
+        src = _c(src, np.float32)
+        tgt = _c(tgt, np.float32)
+        smask = _c(smask, np.uint8)
+        tmask = _c(tmask, np.uint8)
+        sH, sW, C = src.shape
+        tH, tW, _ = tgt.shape
+        dp = DistParams(DIST[kind], patch, sigma, lam, lam_d)
+        pm = PMParams(iters, shots, window, first_step)
+        nnf = np.zeros((tH, tW, 2), np.int32)
+        dist = np.zeros((tH, tW), np.float32) if with_dist else None
+        init = None if init is None else _c(init, np.int32)
+        self._check(self.lib.nli_patchmatch(self.h, ct.byref(dp), ct.byref(pm), _ptr(src, _f32p), _ptr(smask, _u8p),
+                                            sW, sH, _ptr(tgt, _f32p), _ptr(tmask, _u8p), tW, tH, C,
+                                            _ptr(init, _i32p), ct.c_uint64(seed), _ptr(nnf, _i32p),
+                                            _ptr(dist, _f32p)))
+        return (nnf, dist) if with_dist else nnf
+
+    # ---- updates ----
+    def update(self, kind, image, orig, domain, ext, nnf, conf, patch=9, sigma=10000.0, lam=0.05, lam_d=0.05,
+               cg_tol=1e-6, cg_limit=1000, cg_precision=CG_FP64, cg_refine_every=0, stats=False):
+        """AImageUpdating::update: returns (new image, total difference[, CGStats])."""
+        img = np.array(image, dtype=np.float32, copy=True, order="C")
+        orig = _c(orig, np.float32)
+        domain = _c(domain, np.uint8)
+        ext = _c(ext, np.uint8)
+        nnf = _c(nnf, np.int32)
+        conf = None if conf is None else _c(conf, np.float32)
+        H, W, C = img.shape
+        up = UpdParams(UPD[kind], patch, sigma, lam, lam_d, cg_tol, cg_limit, cg_precision, cg_refine_every)
+        total = ct.c_double()
+        st = CGStats()
+        self._check(self.lib.nli_update(self.h, ct.byref(up), _ptr(img, _f32p), _ptr(orig, _f32p),
+                                        _ptr(domain, _u8p), _ptr(ext, _u8p), _ptr(nnf, _i32p), _ptr(conf, _f32p),
+                                        W, H, C, ct.byref(total), ct.byref(st)))
+        return (img, total.value, st) if stats else (img, total.value)
+
+    # ---- full process ----
+    def inpaint(self, image, mask, params: Inpainting):
+        image = _c(image, np.float32)
+        mask = _c(mask, np.uint8)
+        H, W, C = image.shape
+        out = np.zeros_like(image)
+        p = params.to_struct()
+        st = InpaintStats()
+        self._check(self.lib.nli_inpaint(self.h, ct.byref(p), _ptr(image, _f32p), _ptr(mask, _u8p), W, H, C,
+                                         _ptr(out, _f32p), ct.byref(st)))
+        return out, stats_of(st)
+
+
+def stats_of(st: InpaintStats) -> RunStats:
+    s = st.scales
+    return RunStats(s, list(st.iterations[:s]), [(st.width[i], st.height[i]) for i in range(s)],
+                    list(st.hole_pixels[:s]), list(st.target_pixels[:s]), st.cg_iterations_total,
+                    st.seconds_total, st.seconds_patchmatch, st.seconds_update)
+
+
+def poisson_fill(img, mask):
+    """Coarsest-level Poisson fill (host) of the masked pixels of img (H, W, C)."""
+    lib = load_library()
+    out = np.array(img, dtype=np.float32, copy=True, order="C")
+    mask = _c(mask, np.uint8)
+    H, W, C = out.shape
+    rc = lib.nli_poisson_fill(_ptr(out, _f32p), _ptr(mask, _u8p), W, H, C)
+    if rc != NLI_OK:
+        raise NliError(rc, "nli_poisson_fill failed")
+    return out
+
+
+def subsampling_rate(ratio: float, scales: int) -> float:
+    return float(load_library().nli_subsampling_rate(ratio, scales))

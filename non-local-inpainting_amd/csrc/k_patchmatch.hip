@@ -1,0 +1,575 @@
+// k_patchmatch.hip -- gradient, patch distances and PatchMatch NNF search (gfx950).
+//
+// Reference: gradient.cpp:20-76; l2_norm_patch_distance.cpp:22-104,
+// l1_norm_patch_distance.cpp:22-105, l2_combined_patch_distance.cpp:39-148,
+// l2_rgbd_patch_distance.cpp:41-117 (distances); patch_match.cpp:57-302 / 304-512
+// (PatchMatch: initialisation, propagation, random search).
+//
+// Distances keep the reference's float/double types and summation order
+// (-ffp-contract=off), so a distance at a fixed NNF is bit-exact.
+//
+// Search schedule.  The reference's propagation is a sequential scan (each target
+// reads its left/top neighbour updated in the same sweep), which has no parallel
+// equivalent.  Each sweep here is one launch in which every target, in parallel,
+// reads the previous sweep's field (Jacobi) and tries the shifted entries of its four
+// neighbours at distance `step` (jump flooding: step = s0, s0/2, ..., 1, repeated),
+// then runs the reference's random search unchanged: windows max(W,H)/2^k around the
+// post-propagation best, up to `shots` uniform draws per window, the first draw inside
+// the source evaluated.  Candidates are accepted only when strictly better, as in the
+// reference.  A candidate's distance evaluation stops early once its partial sum proves
+// it cannot be accepted; the accept/reject decision is then identical to the full
+// evaluation (all terms are non-negative and rounding is monotone).
+//
+// Work layout.  One thread per target pixel, 32x8 target tiles (one 256-thread block
+// each; only tiles holding a target are launched).  The block stages the target
+// patch data of its tile plus the patch halo in LDS as planar float planes (colour
+// channels, then the 2C gradient planes); source patches are gathered from global
+// memory (L2 / MALL resident), row by row.
+#include "nli_internal.h"
+
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace nli {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int TX = 32;
+constexpr int TY = 8;
+
+// ---- counter-based RNG (SplitMix64 finaliser over a hashed counter) ----
+__device__ inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+struct Rng {
+    uint64_t s;
+    __device__ uint32_t next()
+    {
+        s += 0x9E3779B97F4A7C15ull;
+        return (uint32_t)(mix64(s) >> 32);
+    }
+    // uniform in [0, n) by multiply-high
+    __device__ int below(int n) { return (int)(((uint64_t)next() * (uint32_t)n) >> 32); }
+};
+__device__ inline Rng make_rng(uint64_t seed, uint32_t stream, uint32_t idx)
+{
+    return Rng{mix64(seed ^ mix64(((uint64_t)stream << 32) | idx))};
+}
+
+struct PMArgs {
+    const float *src, *sgrad;
+    const uint8_t *smask;
+    int sW, sH;
+    const float *tgt, *tgrad;
+    const uint8_t *tmask;
+    int tW, tH;
+    int P;
+    const float *w;
+    float lambda, lambda_d;
+    const int2 *nnf_in;
+    const float *dist_in;
+    int2 *nnf_out;
+    float *dist_out;
+    const int *tiles;
+    int ntiles_x;
+    int maxw, shots, step;
+    uint64_t seed;
+    uint32_t stream;
+    const int2 *init;   // initial field (k_pm_init), may be null
+};
+
+// Centres whose patch leaves the image are never targets or sources here (the
+// reference would read outside the image for them; its masks always exclude them).
+__device__ inline bool is_source(const PMArgs &a, int x, int y)
+{
+    int r = a.P / 2;
+    return x >= r && y >= r && x < a.sW - r && y < a.sH - r && a.smask[(size_t)a.sW * y + x];
+}
+__device__ inline bool is_target(const PMArgs &a, int x, int y)
+{
+    int r = a.P / 2;
+    return x >= r && y >= r && x < a.tW - r && y < a.tH - r && a.tmask[(size_t)a.tW * y + x];
+}
+
+template <int KIND>
+struct Feat {
+    static constexpr bool grad = (KIND == DIST_L2COMBINED || KIND == DIST_L2RGBD);
+};
+
+// target patch accessors: f < C colour channel f, f >= C gradient component f - C
+struct TgtLds {
+    const float *lds;
+    int plane, LW, ox, oy;
+    __device__ float v(int f, int dx, int dy) const { return lds[f * plane + (oy + dy) * LW + ox + dx]; }
+};
+template <int C>
+struct TgtGlobal {
+    const float *img, *grad;
+    int W, ox, oy;
+    __device__ float v(int f, int dx, int dy) const
+    {
+        size_t p = (size_t)(oy + dy) * W + (ox + dx);
+        return f < C ? img[p * C + f] : grad[p * 2 * C + (f - C)];
+    }
+};
+
+// Patch distance between the source patch centred at (sx, sy) and the target patch of
+// `t`.  Returns +inf as soon as the partial sum proves the result is >= bd.
+template <int KIND, int C, class T>
+__device__ float patch_distance(const PMArgs &a, const float *__restrict__ w, int sx, int sy, const T &t, float bd)
+{
+    const int P = a.P, r = P / 2;
+    const unsigned PP = (unsigned)(P * P);
+    if constexpr (KIND == DIST_L2 || KIND == DIST_L1) {
+        const float bound = bd * (float)PP * (1.0f + 0x1p-20f);
+        float acc = 0.0f;
+        for (int dy = 0; dy < P; dy++) {
+            const float *srow = a.src + ((size_t)(sy - r + dy) * a.sW + (sx - r)) * C;
+            for (int dx = 0; dx < P; dx++) {
+                float norm = 0.0f;
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    float d = srow[dx * C + c] - t.v(c, dx, dy);
+                    float term = (KIND == DIST_L1) ? fabsf(d) : d * d;
+                    norm = (c == 0) ? term : norm + term;
+                }
+                acc += w[dy * P + dx] * norm;
+            }
+            if (acc > bound) return INFINITY;
+        }
+        return acc / (float)PP;
+    } else {
+        const double bound = (double)bd * (double)PP * (1.0 + 1e-6);
+        const double lam = (double)a.lambda, lam1 = (double)(1 - a.lambda);
+        const double lamd = (double)a.lambda_d, lamd1 = (double)(1 - a.lambda_d);
+        double dist = 0.0, gdist = 0.0, dist_d = 0.0, gdist_d = 0.0;
+        for (int dy = 0; dy < P; dy++) {
+            const size_t rowp = (size_t)(sy - r + dy) * a.sW + (sx - r);
+            const float *srow = a.src + rowp * C;
+            const float *grow = a.sgrad + rowp * 2 * C;
+            for (int dx = 0; dx < P; dx++) {
+                const float wt = w[dy * P + dx];
+                if constexpr (KIND == DIST_L2COMBINED && (C == 1 || C == 3)) {
+                    float norm = 0.0f, gnorm = 0.0f;
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        float d = srow[dx * C + c] - t.v(c, dx, dy);
+                        norm = (c == 0) ? d * d : norm + d * d;
+                    }
+#pragma unroll
+                    for (int k = 0; k < 2 * C; k++) {
+                        float d = grow[dx * 2 * C + k] - t.v(C + k, dx, dy);
+                        gnorm = (k == 0) ? d * d : gnorm + d * d;
+                    }
+                    dist += (double)(wt * norm);
+                    gdist += (double)(wt * gnorm);
+                } else if constexpr (KIND == DIST_L2COMBINED) {
+                    double norm = 0.0, gnorm = 0.0;
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        float d = srow[dx * C + c] - t.v(c, dx, dy);
+                        norm += (double)(d * d);
+                        float gx = grow[dx * 2 * C + 2 * c] - t.v(C + 2 * c, dx, dy);
+                        float gy = grow[dx * 2 * C + 2 * c + 1] - t.v(C + 2 * c + 1, dx, dy);
+                        gnorm += (double)(gx * gx + gy * gy);
+                    }
+                    dist += (double)wt * norm;
+                    gdist += (double)wt * gnorm;
+                } else {   // DIST_L2RGBD, C == 4
+                    double norm = 0.0, norm_d = 0.0, gnorm = 0.0, gnorm_d = 0.0;
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        float d = srow[dx * C + c] - t.v(c, dx, dy);
+                        float gx = grow[dx * 2 * C + 2 * c] - t.v(C + 2 * c, dx, dy);
+                        float gy = grow[dx * 2 * C + 2 * c + 1] - t.v(C + 2 * c + 1, dx, dy);
+                        if (c == 3) {
+                            norm_d += (double)(d * d);
+                            gnorm_d += (double)(gx * gx + gy * gy);
+                        } else {
+                            norm += (double)(d * d);
+                            gnorm += (double)(gx * gx + gy * gy);
+                        }
+                    }
+                    dist += (double)wt * norm;
+                    dist_d += (double)wt * norm_d;
+                    gdist += (double)wt * gnorm;
+                    gdist_d += (double)wt * gnorm_d;
+                }
+            }
+            double partial = lam * dist + lam1 * gdist;
+            if constexpr (KIND == DIST_L2RGBD) partial = partial + lamd * dist_d + lamd1 * gdist_d;
+            if (partial > bound) return INFINITY;
+        }
+        double total = lam * dist + lam1 * gdist;
+        if constexpr (KIND == DIST_L2RGBD) total = total + lamd * dist_d + lamd1 * gdist_d;
+        return (float)(total / (double)PP);
+    }
+}
+
+__global__ void k_gradient(const float *__restrict__ img, int W, int H, int C, float *__restrict__ grad)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W) return;
+    size_t index = (size_t)C * ((size_t)W * y + x);
+    for (int ch = 0; ch < C; ch++) {
+        float v = img[index + ch];
+        grad[(index + ch) * 2] = (x == W - 1) ? 0.0f : img[index + C + ch] - v;
+        grad[(index + ch) * 2 + 1] = (y == H - 1) ? 0.0f : img[index + (size_t)C * W + ch] - v;
+    }
+}
+
+__global__ void k_tile_flags(const uint8_t *tmask, int W, int H, int ntx, int nty, uint8_t *flags)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntx * nty) return;
+    int x0 = (t % ntx) * TX, y0 = (t / ntx) * TY;
+    uint8_t any = 0;
+    for (int y = y0; y < min(y0 + TY, H) && !any; y++)
+        for (int x = x0; x < min(x0 + TX, W); x++)
+            if (tmask[(size_t)W * y + x]) {
+                any = 1;
+                break;
+            }
+    flags[t] = any;
+}
+
+// Initialisation (patch_match.cpp:94-141): the given entry, or uniform draws until one
+// lands in the source (at most `shots`), then its distance.
+template <int KIND, int C>
+__global__ __launch_bounds__(kBlock) void k_pm_init(PMArgs a)
+{
+    __shared__ float wl[15 * 15];
+    for (int i = threadIdx.x; i < a.P * a.P; i += blockDim.x) wl[i] = a.w[i];
+    __syncthreads();
+    int tile = a.tiles[blockIdx.x];
+    int x = (tile % a.ntiles_x) * TX + (int)threadIdx.x % TX;
+    int y = (tile / a.ntiles_x) * TY + (int)threadIdx.x / TX;
+    if (x >= a.tW || y >= a.tH || !is_target(a, x, y)) return;
+    size_t p = (size_t)a.tW * y + x;
+    Rng rng = make_rng(a.seed, a.stream, (uint32_t)p);
+    int cx, cy, tries = 0;
+    if (a.init) {
+        int2 c = a.init[p];
+        cx = c.x;
+        cy = c.y;
+        while (!is_source(a, cx, cy) && tries < a.shots) {
+            cx = rng.below(a.sW);
+            cy = rng.below(a.sH);
+            tries++;
+        }
+    } else {
+        cx = -1;
+        cy = -1;
+        while (!is_source(a, cx, cy) && tries < a.shots) {
+            cx = rng.below(a.sW);
+            cy = rng.below(a.sH);
+            tries++;
+        }
+    }
+    int r = a.P / 2;
+    if (is_source(a, cx, cy)) {
+        TgtGlobal<C> t{a.tgt, a.tgrad, a.tW, x - r, y - r};
+        a.nnf_out[p] = make_int2(cx, cy);
+        a.dist_out[p] = patch_distance<KIND, C>(a, wl, cx, cy, t, FLT_MAX);
+    }
+}
+
+// One sweep: jump-flood propagation at distance `step` + random search.
+template <int KIND, int C>
+__global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
+{
+    extern __shared__ float lds[];
+    constexpr int F = Feat<KIND>::grad ? 3 * C : C;
+    const int r = a.P / 2;
+    const int LW = TX + 2 * r, LH = TY + 2 * r;
+    const int plane = LW * LH;
+    float *wl = lds + F * plane;
+    const int tile = a.tiles[blockIdx.x];
+    const int x0 = (tile % a.ntiles_x) * TX, y0 = (tile / a.ntiles_x) * TY;
+    for (int i = threadIdx.x; i < a.P * a.P; i += blockDim.x) wl[i] = a.w[i];
+    for (int i = threadIdx.x; i < plane; i += blockDim.x) {
+        int gx = x0 - r + i % LW, gy = y0 - r + i / LW;
+        bool in = (unsigned)gx < (unsigned)a.tW && (unsigned)gy < (unsigned)a.tH;
+        size_t q = in ? (size_t)gy * a.tW + gx : 0;
+#pragma unroll
+        for (int c = 0; c < C; c++) lds[c * plane + i] = in ? a.tgt[q * C + c] : 0.0f;
+        if constexpr (F > C) {
+#pragma unroll
+            for (int k = 0; k < 2 * C; k++) lds[(C + k) * plane + i] = in ? a.tgrad[q * 2 * C + k] : 0.0f;
+        }
+    }
+    __syncthreads();
+    const int lx = (int)threadIdx.x % TX, ly = (int)threadIdx.x / TX;
+    const int x = x0 + lx, y = y0 + ly;
+    if (x >= a.tW || y >= a.tH || !is_target(a, x, y)) return;
+    const size_t p = (size_t)a.tW * y + x;
+    const TgtLds t{lds, plane, LW, lx, ly};
+    int2 best = a.nnf_in[p];
+    float bd = a.dist_in[p];
+
+    const int s = a.step;
+    const int ox[4] = {-s, s, 0, 0};
+    const int oy[4] = {0, 0, -s, s};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int qx = x + ox[k], qy = y + oy[k];
+        if (!is_target(a, qx, qy)) continue;
+        int2 c = a.nnf_in[(size_t)a.tW * qy + qx];
+        c.x -= ox[k];
+        c.y -= oy[k];
+        if (!is_source(a, c.x, c.y)) continue;
+        float d = patch_distance<KIND, C>(a, wl, c.x, c.y, t, bd);
+        if (d < bd) {
+            bd = d;
+            best = c;
+        }
+    }
+
+    Rng rng = make_rng(a.seed, a.stream, (uint32_t)p);
+    const int scx = best.x, scy = best.y;
+    for (int ws = a.maxw; ws >= 1; ws /= 2) {
+        int x_min = max(scx - ws, 0), y_min = max(scy - ws, 0);
+        int x_max = min(scx + ws + 1, a.sW), y_max = min(scy + ws + 1, a.sH);
+        int cx = -1, cy = -1;
+        bool found = false;
+        for (int k = 0; k < a.shots; k++) {
+            cx = x_min + rng.below(x_max - x_min);
+            cy = y_min + rng.below(y_max - y_min);
+            if (is_source(a, cx, cy)) {
+                found = true;
+                break;
+            }
+        }
+        if (found) {
+            float d = patch_distance<KIND, C>(a, wl, cx, cy, t, bd);
+            if (d < bd) {
+                bd = d;
+                best = make_int2(cx, cy);
+            }
+        }
+    }
+    a.nnf_out[p] = best;
+    a.dist_out[p] = bd;
+}
+
+template <int KIND, int C>
+__global__ __launch_bounds__(kBlock) void k_dist_field(PMArgs a, const int2 *nnf)
+{
+    __shared__ float wl[15 * 15];
+    for (int i = threadIdx.x; i < a.P * a.P; i += blockDim.x) wl[i] = a.w[i];
+    __syncthreads();
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= a.tW) return;
+    size_t p = (size_t)a.tW * y + x;
+    float out = FLT_MAX;
+    if (is_target(a, x, y)) {
+        int2 c = nnf[p];
+        if (c.x >= 0 && (unsigned)c.x < (unsigned)a.sW && (unsigned)c.y < (unsigned)a.sH) {
+            int r = a.P / 2;
+            if (c.x >= r && c.y >= r && c.x < a.sW - r && c.y < a.sH - r) {
+                TgtGlobal<C> t{a.tgt, a.tgrad, a.tW, x - r, y - r};
+                out = patch_distance<KIND, C>(a, wl, c.x, c.y, t, FLT_MAX);
+            }
+        }
+    }
+    a.dist_out[p] = out;
+}
+
+// ---- dispatch over (kind, channels) ----
+template <template <int, int> class K>
+struct Dispatch;
+
+#define NLI_DISPATCH(KIND, C, CALL)                                                           \
+    switch (KIND) {                                                                           \
+    case DIST_L2:                                                                             \
+        switch (C) {                                                                          \
+        case 1: CALL(DIST_L2, 1); break;                                                      \
+        case 2: CALL(DIST_L2, 2); break;                                                      \
+        case 3: CALL(DIST_L2, 3); break;                                                      \
+        case 4: CALL(DIST_L2, 4); break;                                                      \
+        default: throw std::invalid_argument("patch distance: 1 to 4 channels");              \
+        }                                                                                     \
+        break;                                                                                \
+    case DIST_L1:                                                                             \
+        switch (C) {                                                                          \
+        case 1: CALL(DIST_L1, 1); break;                                                      \
+        case 2: CALL(DIST_L1, 2); break;                                                      \
+        case 3: CALL(DIST_L1, 3); break;                                                      \
+        case 4: CALL(DIST_L1, 4); break;                                                      \
+        default: throw std::invalid_argument("patch distance: 1 to 4 channels");              \
+        }                                                                                     \
+        break;                                                                                \
+    case DIST_L2COMBINED:                                                                     \
+        switch (C) {                                                                          \
+        case 1: CALL(DIST_L2COMBINED, 1); break;                                              \
+        case 2: CALL(DIST_L2COMBINED, 2); break;                                              \
+        case 3: CALL(DIST_L2COMBINED, 3); break;                                              \
+        case 4: CALL(DIST_L2COMBINED, 4); break;                                              \
+        default: throw std::invalid_argument("patch distance: 1 to 4 channels");              \
+        }                                                                                     \
+        break;                                                                                \
+    case DIST_L2RGBD:                                                                         \
+        if (C != 4) throw std::invalid_argument("L2 RGBD Distance: Channels do not match.");  \
+        CALL(DIST_L2RGBD, 4);                                                                 \
+        break;                                                                                \
+    default: throw std::invalid_argument("unknown patch distance kind");                      \
+    }
+
+struct Prepared {
+    PMArgs a;
+    int ntiles = 0;
+};
+
+Prepared prepare(const DistDesc &d, const float *src, const uint8_t *smask, int sW, int sH, const float *tgt,
+                 const uint8_t *tmask, int tW, int tH, int C, PMWork &w, hipStream_t s, bool need_tiles)
+{
+    if (d.patch < 1 || d.patch > 15) throw std::invalid_argument("patch side must be in [1, 15]");
+    if (C < 1 || C > 4) throw std::invalid_argument("1 to 4 channels supported");
+    Prepared pr;
+    PMArgs &a = pr.a;
+    a = PMArgs{};
+    a.src = src;
+    a.smask = smask;
+    a.sW = sW;
+    a.sH = sH;
+    a.tgt = tgt;
+    a.tmask = tmask;
+    a.tW = tW;
+    a.tH = tH;
+    a.P = d.patch;
+    a.lambda = d.lambda;
+    a.lambda_d = d.lambda_d;
+    std::vector<float> wh((size_t)d.patch * d.patch);
+    gaussian_weights(d.patch, d.patch, d.sigma, d.sigma, wh.data());
+    w.weights.ensure(wh.size());
+    NLI_HIP(hipMemcpyAsync(w.weights.p, wh.data(), wh.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    NLI_HIP(hipStreamSynchronize(s));   // wh is pageable and local
+    a.w = w.weights.p;
+    if (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) {
+        w.sgrad.ensure((size_t)sW * sH * C * 2);
+        gradient(src, sW, sH, C, w.sgrad.p, s);
+        a.sgrad = w.sgrad.p;
+        if (tgt == src) {
+            a.tgrad = w.sgrad.p;
+        } else {
+            w.tgrad.ensure((size_t)tW * tH * C * 2);
+            gradient(tgt, tW, tH, C, w.tgrad.p, s);
+            a.tgrad = w.tgrad.p;
+        }
+    }
+    if (need_tiles) {
+        int ntx = (tW + TX - 1) / TX, nty = (tH + TY - 1) / TY;
+        int nt = ntx * nty;
+        w.tflags.ensure(nt);
+        k_tile_flags<<<(nt + kBlock - 1) / kBlock, kBlock, 0, s>>>(tmask, tW, tH, ntx, nty, w.tflags.p);
+        NLI_LAUNCH_CHECK();
+        w.tiles.ensure(nt);
+        w.counts.ensure(nt / 1024 + 2);
+        w.total.ensure(1);
+        pr.ntiles = compact_indices(w.tflags.p, nt, w.tiles.p, w.counts.p, w.total.p, s);
+        a.tiles = w.tiles.p;
+        a.ntiles_x = ntx;
+    }
+    return pr;
+}
+
+int auto_first_step(int W, int H)
+{
+    int m = std::max(W, H) / 32;
+    int s = 1;
+    while (s * 2 <= m && s < 32) s *= 2;
+    return s;
+}
+
+}  // namespace
+
+void gradient(const float *img, int W, int H, int C, float *grad, hipStream_t s)
+{
+    dim3 g((W + kBlock - 1) / kBlock, H);
+    k_gradient<<<g, kBlock, 0, s>>>(img, W, H, C, grad);
+    NLI_LAUNCH_CHECK();
+}
+
+void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const uint8_t *smask, int sW, int sH,
+                const float *tgt, const uint8_t *tmask, int tW, int tH, int C, const int32_t *init_nnf,
+                uint64_t seed, int32_t *out_nnf, float *out_dist, PMWork &w, hipStream_t s)
+{
+    Prepared pr = prepare(d, src, smask, sW, sH, tgt, tmask, tW, tH, C, w, s, true);
+    PMArgs a = pr.a;
+    size_t tn = (size_t)tW * tH;
+    float *dA = out_dist ? out_dist : w.dist0.ensure(tn);
+    int32_t *nB = w.nnf1.ensure(2 * tn);
+    float *dB = w.dist1.ensure(tn);
+    // the final sweep must land in (out_nnf, dA)
+    int K = std::max(cfg.iterations, 0);
+    int2 *n_cur = (int2 *)((K % 2 == 0) ? out_nnf : nB);
+    float *d_cur = (K % 2 == 0) ? dA : dB;
+    int2 *n_oth = (int2 *)((K % 2 == 0) ? nB : out_nnf);
+    float *d_oth = (K % 2 == 0) ? dB : dA;
+    fill_i32((int32_t *)n_cur, -1, 2 * tn, s);
+    fill_f32(d_cur, FLT_MAX, tn, s);
+    if (K > 0) {
+        fill_i32((int32_t *)n_oth, -1, 2 * tn, s);
+        fill_f32(d_oth, FLT_MAX, tn, s);
+    }
+    if (pr.ntiles == 0) {
+        if (K % 2 == 1) {   // result must still be in out_nnf / dA
+            NLI_HIP(hipMemcpyAsync(out_nnf, n_cur, 2 * tn * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+            NLI_HIP(hipMemcpyAsync(dA, d_cur, tn * sizeof(float), hipMemcpyDeviceToDevice, s));
+        }
+        return;
+    }
+    a.shots = cfg.shots;
+    a.maxw = (cfg.window != -1) ? cfg.window : std::max(sW, sH);
+    a.seed = seed;
+    a.init = (const int2 *)init_nnf;
+    a.nnf_out = n_cur;
+    a.dist_out = d_cur;
+    a.stream = 0xFFFFu;
+#define NLI_INIT(KD, CC) k_pm_init<KD, CC><<<pr.ntiles, kBlock, 0, s>>>(a)
+    NLI_DISPATCH(d.kind, C, NLI_INIT);
+#undef NLI_INIT
+    NLI_LAUNCH_CHECK();
+    const int r = d.patch / 2;
+    const int F = (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) ? 3 * C : C;
+    const size_t lds = ((size_t)F * (TX + 2 * r) * (TY + 2 * r) + (size_t)d.patch * d.patch) * sizeof(float);
+    const int s0 = cfg.first_step > 0 ? cfg.first_step : auto_first_step(tW, tH);
+    int levels = 0;
+    while ((s0 >> levels) > 1) levels++;
+    for (int k = 0; k < K; k++) {
+        a.nnf_in = n_cur;
+        a.dist_in = d_cur;
+        a.nnf_out = n_oth;
+        a.dist_out = d_oth;
+        a.step = s0 >> (k % (levels + 1));
+        a.stream = (uint32_t)k;
+#define NLI_SWEEP(KD, CC) k_pm_sweep<KD, CC><<<pr.ntiles, kBlock, lds, s>>>(a)
+        NLI_DISPATCH(d.kind, C, NLI_SWEEP);
+#undef NLI_SWEEP
+        NLI_LAUNCH_CHECK();
+        std::swap(n_cur, n_oth);
+        std::swap(d_cur, d_oth);
+    }
+}
+
+void patch_distance_field(const DistDesc &d, const float *src, int sW, int sH, const float *tgt, int tW, int tH,
+                          int C, const int32_t *nnf, const uint8_t *tmask, float *out, PMWork &w, hipStream_t s)
+{
+    // the source mask is not needed for a fixed field; pass the target mask (unused)
+    Prepared pr = prepare(d, src, tmask, sW, sH, tgt, tmask, tW, tH, C, w, s, false);
+    PMArgs a = pr.a;
+    a.dist_out = out;
+    dim3 g((tW + kBlock - 1) / kBlock, tH);
+#define NLI_FIELD(KD, CC) k_dist_field<KD, CC><<<g, kBlock, 0, s>>>(a, (const int2 *)nnf)
+    NLI_DISPATCH(d.kind, C, NLI_FIELD);
+#undef NLI_FIELD
+    NLI_LAUNCH_CHECK();
+}
+
+}  // namespace nli

@@ -1,0 +1,940 @@
+// k_update.hip -- the non-local image updates (gfx950): NL-means, NL-medians,
+// NL-Poisson (coefficients + conjugate gradient) and NL-Depth (two NL-Poisson
+// solves on channel subsets).
+//
+// Reference: patch_non_local_means.cpp:23-100, patch_non_local_medians.cpp:23-126,
+// patch_non_local_poisson.cpp:36-477, patch_non_local_depth.cpp:36-55.
+//
+// * Means/medians: one thread per hole pixel gathers its clipped P x P window in the
+//   reference's order (x outer, y inner), float sums as in the reference.  New values
+//   go to a side buffer and are written back afterwards, so the result does not depend
+//   on thread order (the reference updates in place in scan order; for NNF entries that
+//   are source points the two are identical, contributors never being hole pixels).
+// * Poisson coefficients: the reference scatters from every extended-domain centre p
+//   into the P x P pixels q around it.  Here each pixel q gathers from the centres p
+//   around it, visiting them in row-major order -- the order in which the scatter
+//   reaches q -- so the double sums a1, a2, f1, F1, F2 are bit-exact.
+// * CG: the reference recurrence (r recomputed as b - A x each iteration, stop on
+//   |r|^2 < tol |r0|^2 or iteration >= limit).  Unknowns are the hole pixels in
+//   row-major order; A x = a1 x - L x with the reference's anisotropic 5-point
+//   stencil.  Two kernels per iteration for all channels at once; the direction and
+//   iterate updates of the neighbours are recomputed on the fly (double-buffered), and
+//   each reduction finishes in the last block to arrive (fixed summation order, so the
+//   run is deterministic).  Scalars (alpha, beta, stop flags) stay on the device.
+// * Mixed precision (precision = 1): the same recurrence with FP32 vectors and FP32
+//   stencil arithmetic, FP64 dot-product accumulation, and the residual recomputed
+//   from an FP64 evaluation of b - A x every `refine_every` iterations.
+#include "nli_internal.h"
+
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace nli {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxNC = 4;
+constexpr int kMaxP = 15;
+
+// scalar slots per channel
+enum { S_NR0 = 0, S_NR, S_RDOTQ, S_ALPHA, S_BETA, S_DONE, S_ITER, S_NPART, S_COUNT };
+
+// ---------------- deterministic block reductions ----------------
+template <int NC>
+__device__ inline void block_reduce_store(double (&v)[NC], double *partials, int nb)
+{
+    __shared__ double sh[NC][kBlock];
+    for (int c = 0; c < NC; c++) sh[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int c = 0; c < NC; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NC; c++) partials[(size_t)c * nb + blockIdx.x] = sh[c][0];
+}
+
+// After every block stored its partials: the last block to arrive sums them (in block
+// order) and returns true with the totals in tot[] (valid in thread 0).
+template <int NC>
+__device__ inline bool last_block_total(const double *partials, int nb, unsigned *counter, double (&tot)[NC])
+{
+    __shared__ bool is_last;
+    __threadfence();
+    if (threadIdx.x == 0) is_last = (atomicAdd(counter, 1u) == (unsigned)(nb - 1));
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    __shared__ double sh[NC][kBlock];
+    for (int c = 0; c < NC; c++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) s += ((volatile const double *)partials)[(size_t)c * nb + b];
+        sh[c][threadIdx.x] = s;
+    }
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int c = 0; c < NC; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + off];
+        __syncthreads();
+    }
+    for (int c = 0; c < NC; c++) tot[c] = sh[c][0];
+    if (threadIdx.x == 0) *counter = 0u;
+    return true;
+}
+
+// ---------------- NL-means ----------------
+__global__ void k_nlmeans(const float *__restrict__ img, int Cs, int c0, int nc, const int *__restrict__ pts,
+                          int n, const uint8_t *__restrict__ hole, const int2 *__restrict__ nnf,
+                          const float *__restrict__ conf, const float *__restrict__ w, int P, int W, int H,
+                          float *__restrict__ newv, uint8_t *__restrict__ has)
+{
+    __shared__ float wl[kMaxP * kMaxP];
+    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
+    __syncthreads();
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int p = pts[k];
+    int x = p % W, y = p / W;
+    int h = P / 2;
+    int x_a = max(x - h, 0), x_b = min(x + h, W - 1);
+    int y_a = max(y - h, 0), y_b = min(y + h, H - 1);
+    float tv[kMaxNC] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float tw = 0.0f;
+    for (int i = x_a; i <= x_b; i++)
+        for (int j = y_a; j <= y_b; j++) {
+            size_t q = (size_t)W * j + i;
+            int2 c = nnf[q];
+            if (c.x == -1) continue;
+            c.x += x - i;
+            c.y += y - j;
+            if (c.x < 0 || c.y < 0 || c.x >= W || c.y >= H) continue;
+            float wt = wl[P * (j - y_a) + (i - x_a)];
+            if (conf && hole[q]) wt *= conf[q];
+            tw += wt;
+            const float *src = img + ((size_t)W * c.y + c.x) * Cs + c0;
+            for (int ch = 0; ch < kMaxNC; ch++)
+                if (ch < nc) tv[ch] += src[ch] * wt;
+        }
+    has[k] = tw > 0.0;
+    if (tw > 0.0)
+        for (int ch = 0; ch < nc; ch++) newv[(size_t)k * nc + ch] = tv[ch] / tw;
+}
+
+// ---------------- NL-medians ----------------
+// Per hole pixel and channel: (value, weight) pairs inserted in value order into this
+// thread's LDS column (equal values keep gather order), then the weights are subtracted
+// from the total in that order until it drops below half (compute_weighted_median).
+__global__ void k_nlmedians(const float *__restrict__ img, int Cs, int c0, int nc, const int *__restrict__ pts,
+                            int n, const uint8_t *__restrict__ hole, const int2 *__restrict__ nnf,
+                            const float *__restrict__ conf, const float *__restrict__ w, int P, int W, int H,
+                            float *__restrict__ newv, uint8_t *__restrict__ has)
+{
+    extern __shared__ float med_lds[];
+    const int PP = P * P;
+    const int bs = blockDim.x;
+    float *wl = med_lds;                       // PP weights
+    float *sv = med_lds + PP;                  // PP x bs values
+    float *so = sv + (size_t)PP * bs;          // PP x bs weights
+    for (int i = threadIdx.x; i < PP; i += bs) wl[i] = w[i];
+    __syncthreads();
+    int k = blockIdx.x * bs + threadIdx.x;
+    if (k >= n) return;
+    const int t = threadIdx.x;
+    int p = pts[k];
+    int x = p % W, y = p / W;
+    int h = P / 2;
+    int x_a = max(x - h, 0), x_b = min(x + h, W - 1);
+    int y_a = max(y - h, 0), y_b = min(y + h, H - 1);
+    for (int ch = 0; ch < nc; ch++) {
+        float tw = 0.0f;
+        int m = 0;
+        for (int i = x_a; i <= x_b; i++)
+            for (int j = y_a; j <= y_b; j++) {
+                size_t q = (size_t)W * j + i;
+                int2 c = nnf[q];
+                if (c.x == -1) continue;
+                c.x += x - i;
+                c.y += y - j;
+                if (c.x < 0 || c.y < 0 || c.x >= W || c.y >= H) continue;
+                float wt = wl[P * (j - y_a) + (i - x_a)];
+                if (conf && hole[q]) wt *= conf[q];
+                tw += wt;
+                float v = img[((size_t)W * c.y + c.x) * Cs + c0 + ch];
+                int pos = m;
+                while (pos > 0 && sv[(pos - 1) * bs + t] > v) {
+                    sv[pos * bs + t] = sv[(pos - 1) * bs + t];
+                    so[pos * bs + t] = so[(pos - 1) * bs + t];
+                    pos--;
+                }
+                sv[pos * bs + t] = v;
+                so[pos * bs + t] = wt;
+                m++;
+            }
+        if (ch == 0) has[k] = m > 0;
+        if (m == 0) return;
+        float half = tw / 2;
+        float total = tw;
+        float med = 0.0f;
+        for (int e = 0; e < m; e++) {
+            total -= so[e * bs + t];
+            if (total < half) {
+                med = sv[e * bs + t];
+                break;
+            }
+        }
+        newv[(size_t)k * nc + ch] = med;
+    }
+}
+
+// write new values into the hole pixels that received some, and sum (prev - new)^2
+__global__ void k_write_back(float *__restrict__ img, int Cs, int c0, int nc, const int *__restrict__ pts, int n,
+                             const float *__restrict__ newv, const uint8_t *__restrict__ has,
+                             double *__restrict__ partials, unsigned *counter, double *result)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[1] = {0.0};
+    if (k < n && has[k]) {
+        float *px = img + (size_t)pts[k] * Cs + c0;
+        for (int ch = 0; ch < nc; ch++) {
+            float v = newv[(size_t)k * nc + ch];
+            float d = px[ch] - v;
+            acc[0] += (double)(d * d);
+            px[ch] = v;
+        }
+    }
+    int nb = gridDim.x;
+    block_reduce_store<1>(acc, partials, nb);
+    double tot[1];
+    if (last_block_total<1>(partials, nb, counter, tot) && threadIdx.x == 0) *result = tot[0];
+}
+
+// ---------------- NL-Poisson coefficients ----------------
+// a1, a2: W*H; f1, F1, F2: nc planes of W*H (double)
+__global__ void k_poisson_coef(const float *__restrict__ img, const float *__restrict__ grad, int Cs, int c0,
+                               int nc, const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
+                               const float *__restrict__ conf, const float *__restrict__ w, int P, float lambda,
+                               int W, int H, double *__restrict__ a1, double *__restrict__ a2,
+                               double *__restrict__ f1, double *__restrict__ F1, double *__restrict__ F2)
+{
+    __shared__ float wl[kMaxP * kMaxP];
+    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
+    __syncthreads();
+    int qx = blockIdx.x * blockDim.x + threadIdx.x;
+    int qy = blockIdx.y;
+    if (qx >= W) return;
+    const int h = P / 2;
+    const size_t plane = (size_t)W * H;
+    const size_t q = (size_t)W * qy + qx;
+    const double lam = (double)lambda;
+    const double lam1 = (double)(1 - lambda);
+    double A1 = 0.0, A2 = 0.0;
+    double v1[kMaxNC] = {0, 0, 0, 0}, g1[kMaxNC] = {0, 0, 0, 0}, g2[kMaxNC] = {0, 0, 0, 0};
+    for (int py = max(qy - h, 0); py <= min(qy + h, H - 1); py++)
+        for (int px = max(qx - h, 0); px <= min(qx + h, W - 1); px++) {
+            size_t pidx = (size_t)W * py + px;
+            if (!ext[pidx]) continue;
+            int2 nb = nnf[pidx];
+            int n_x = nb.x + qx - px, n_y = nb.y + qy - py;
+            if (n_x < 0 || n_y < 0 || n_x >= W || n_y >= H) continue;
+            float confidence = conf ? conf[pidx] : 1.0f;
+            double weight = (double)(confidence * wl[P * (qy - py + h) + (qx - px + h)]);
+            A1 += weight * lam;
+            A2 += weight * lam1;
+            size_t nidx = (size_t)W * n_y + n_x;
+            const float *iv = img + nidx * Cs + c0;
+            const float *gv = grad + (nidx * Cs + c0) * 2;
+            for (int ch = 0; ch < kMaxNC; ch++)
+                if (ch < nc) {
+                    v1[ch] += weight * (double)iv[ch] * lam;
+                    g1[ch] += weight * (double)gv[2 * ch] * lam1;
+                    g2[ch] += weight * (double)gv[2 * ch + 1] * lam1;
+                }
+        }
+    a1[q] = A1;
+    a2[q] = A2;
+    for (int ch = 0; ch < nc; ch++) {
+        f1[ch * plane + q] = v1[ch];
+        F1[ch * plane + q] = g1[ch];
+        F2[ch * plane + q] = g2[ch];
+    }
+}
+
+// backward divergence of (F1, F2) at extended-domain points (patch_non_local_poisson.cpp:235-285)
+__global__ void k_divergence(const uint8_t *__restrict__ ext, const double *__restrict__ F1,
+                             const double *__restrict__ F2, int nc, int W, int H, int active,
+                             double *__restrict__ f2)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W) return;
+    const size_t plane = (size_t)W * H;
+    const size_t idx = (size_t)W * y + x;
+    for (int ch = 0; ch < nc; ch++) {
+        const double *fx = F1 + ch * plane, *fy = F2 + ch * plane;
+        double dv = 0.0;
+        if (active && ext[idx]) {
+            if (x == 0) dv = fx[idx];
+            else if (x == W - 1) dv = -fx[idx - 1];
+            else dv = fx[idx] - fx[idx - 1];
+            if (y == 0) dv += fy[idx];
+            else if (y == H - 1) dv += -fy[idx - W];
+            else dv += fy[idx] - fy[idx - W];
+        }
+        f2[ch * plane + idx] = dv;
+    }
+}
+
+// ---------------- CG set-up ----------------
+// stencil per hole point: a_t, a_l, a_b, a_r, sum, a1  (reference coefficient choice:
+// top/left use a2 of that neighbour, bottom/right a2 of the centre; 0 outside the image)
+struct Stencil {
+    double at, al, ab, ar, sum, a1;
+};
+
+__global__ void k_lidx(const int *pts, int n, int *lidx)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) lidx[pts[k]] = k;
+}
+
+__global__ void k_cg_setup(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
+                           const double *__restrict__ a1, const double *__restrict__ a2, int W, int H,
+                           Stencil *__restrict__ st, int4 *__restrict__ nbr)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int p = pts[k];
+    int x = p % W, y = p / W;
+    Stencil s;
+    int4 nb;
+    if (y != 0) { s.at = a2[p - W]; nb.x = lidx[p - W]; } else { s.at = 0; nb.x = -1; }
+    if (x != 0) { s.al = a2[p - 1]; nb.y = lidx[p - 1]; } else { s.al = 0; nb.y = -1; }
+    if (y != H - 1) { s.ab = a2[p]; nb.z = lidx[p + W]; } else { s.ab = 0; nb.z = -1; }
+    if (x != W - 1) { s.ar = a2[p]; nb.w = lidx[p + 1]; } else { s.ar = 0; nb.w = -1; }
+    s.sum = s.at + s.al + s.ab + s.ar;
+    s.a1 = a1[p];
+    st[k] = s;
+    nbr[k] = nb;
+}
+
+// anisotropic Laplacian of a list-space vector (0 outside the hole) at point k
+__device__ inline double lap(const Stencil &s, const int4 &nb, const double *v, double vc)
+{
+    double ut = nb.x >= 0 ? v[nb.x] : 0.0;
+    double ul = nb.y >= 0 ? v[nb.y] : 0.0;
+    double ub = nb.z >= 0 ? v[nb.z] : 0.0;
+    double ur = nb.w >= 0 ? v[nb.w] : 0.0;
+    return ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - vc * s.sum;
+}
+
+// b = f1 - f2 - a1 u0 + L(u0) with u0 over the full image; x0 = image - u0
+__global__ void k_cg_rhs(const int *__restrict__ pts, int n, const float *__restrict__ img,
+                         const float *__restrict__ orig, int Cs, int c0, int nc, const double *__restrict__ a1,
+                         const double *__restrict__ a2, const double *__restrict__ f1,
+                         const double *__restrict__ f2, int W, int H, double *__restrict__ b,
+                         double *__restrict__ x0)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int p = pts[k];
+    int x = p % W, y = p / W;
+    const size_t plane = (size_t)W * H;
+    double at, al, ab, ar;
+    at = (y != 0) ? a2[p - W] : 0.0;
+    al = (x != 0) ? a2[p - 1] : 0.0;
+    ab = (y != H - 1) ? a2[p] : 0.0;
+    ar = (x != W - 1) ? a2[p] : 0.0;
+    for (int ch = 0; ch < nc; ch++) {
+        auto U = [&](int q) { return (double)orig[(size_t)q * Cs + c0 + ch]; };
+        double ut = (y != 0) ? U(p - W) : 0.0;
+        double ul = (x != 0) ? U(p - 1) : 0.0;
+        double ub = (y != H - 1) ? U(p + W) : 0.0;
+        double ur = (x != W - 1) ? U(p + 1) : 0.0;
+        double u0 = U(p);
+        double L = ut * at + ul * al + ub * ab + ur * ar - u0 * (at + al + ab + ar);
+        double rhs = f1[ch * plane + p] + (-f2[ch * plane + p] - a1[p] * u0 + L);
+        b[(size_t)ch * n + k] = rhs;
+        x0[(size_t)ch * n + k] = (double)img[(size_t)p * Cs + c0 + ch] - u0;
+    }
+}
+
+struct CGArgs {
+    const Stencil *st;
+    const int4 *nbr;
+    const double *b;
+    int n, nc, nb;
+    double *x[2];
+    double *d[2];
+    double *r;
+    double *partials;
+    unsigned *counter;
+    double *scal;   // S_COUNT per channel
+    float tol;
+    int limit;
+};
+
+// r0 = b - A x0, d0 = r0, nr0 = |r0|^2
+template <int NC>
+__global__ void k_cg_init(CGArgs a)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[NC];
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
+    if (k < a.n) {
+        Stencil s = a.st[k];
+        int4 nb = a.nbr[k];
+        for (int c = 0; c < NC; c++) {
+            const double *xv = a.x[0] + (size_t)c * a.n;
+            double xc = xv[k];
+            double rv = a.b[(size_t)c * a.n + k] - s.a1 * xc + lap(s, nb, xv, xc);
+            a.r[(size_t)c * a.n + k] = rv;
+            a.d[0][(size_t)c * a.n + k] = rv;
+            acc[c] = rv * rv;
+        }
+    }
+    block_reduce_store<NC>(acc, a.partials, a.nb);
+    double tot[NC];
+    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
+        for (int c = 0; c < NC; c++) {
+            double *sc = a.scal + c * S_COUNT;
+            sc[S_NR0] = tot[c];
+            sc[S_NR] = tot[c];
+            sc[S_DONE] = (tot[c] > 0) ? 0.0 : 1.0;
+            sc[S_ITER] = 0.0;
+        }
+}
+
+// iteration `it`, phase A: d_it = r + beta d_{it-1} (it > 0), rdotq = <r, A d_it>, alpha
+template <int NC>
+__global__ void k_cg_a(CGArgs a, int it)
+{
+    __shared__ double sdone[NC], sbeta[NC];
+    if (threadIdx.x < NC) {
+        sdone[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_DONE];
+        sbeta[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_BETA];
+    }
+    __syncthreads();
+    bool all_done = true;
+    for (int c = 0; c < NC; c++) all_done = all_done && sdone[c] != 0.0;
+    if (all_done) return;
+    const double *dp = a.d[it & 1];
+    double *dn = a.d[(it + 1) & 1];
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[NC];
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
+    if (k < a.n) {
+        Stencil s = a.st[k];
+        int4 nb = a.nbr[k];
+        for (int c = 0; c < NC; c++) {
+            if (sdone[c] != 0.0) continue;
+            size_t o = (size_t)c * a.n;
+            const double *rv = a.r + o, *dv = dp + o;
+            double beta = sbeta[c];
+            auto D = [&](int j) { return it == 0 ? dv[j] : rv[j] + beta * dv[j]; };
+            double dc = D(k);
+            double ut = nb.x >= 0 ? D(nb.x) : 0.0;
+            double ul = nb.y >= 0 ? D(nb.y) : 0.0;
+            double ub = nb.z >= 0 ? D(nb.z) : 0.0;
+            double ur = nb.w >= 0 ? D(nb.w) : 0.0;
+            double L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - dc * s.sum;
+            dn[o + k] = dc;
+            acc[c] = rv[k] * (s.a1 * dc - L);
+        }
+    }
+    block_reduce_store<NC>(acc, a.partials, a.nb);
+    double tot[NC];
+    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
+        for (int c = 0; c < NC; c++) {
+            double *sc = a.scal + c * S_COUNT;
+            if (sc[S_DONE] != 0.0) continue;
+            sc[S_RDOTQ] = tot[c];
+            sc[S_ALPHA] = sc[S_NR] / tot[c];
+        }
+}
+
+// phase B: x_{it+1} = x_it + alpha d_it, r = b - A x_{it+1}, nr, beta, stop test
+template <int NC>
+__global__ void k_cg_b(CGArgs a, int it)
+{
+    __shared__ double sdone[NC], salpha[NC];
+    if (threadIdx.x < NC) {
+        sdone[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_DONE];
+        salpha[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_ALPHA];
+    }
+    __syncthreads();
+    bool all_done = true;
+    for (int c = 0; c < NC; c++) all_done = all_done && sdone[c] != 0.0;
+    if (all_done) return;
+    const double *xp = a.x[it & 1];
+    double *xn = a.x[(it + 1) & 1];
+    const double *dcur = a.d[(it + 1) & 1];
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[NC];
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
+    if (k < a.n) {
+        Stencil s = a.st[k];
+        int4 nb = a.nbr[k];
+        for (int c = 0; c < NC; c++) {
+            if (sdone[c] != 0.0) continue;
+            size_t o = (size_t)c * a.n;
+            const double *xv = xp + o, *dv = dcur + o;
+            double alpha = salpha[c];
+            auto X = [&](int j) { return xv[j] + alpha * dv[j]; };
+            double xc = X(k);
+            double ut = nb.x >= 0 ? X(nb.x) : 0.0;
+            double ul = nb.y >= 0 ? X(nb.y) : 0.0;
+            double ub = nb.z >= 0 ? X(nb.z) : 0.0;
+            double ur = nb.w >= 0 ? X(nb.w) : 0.0;
+            double L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - xc * s.sum;
+            double rv = a.b[o + k] - s.a1 * xc + L;
+            xn[o + k] = xc;
+            a.r[o + k] = rv;
+            acc[c] = rv * rv;
+        }
+    }
+    block_reduce_store<NC>(acc, a.partials, a.nb);
+    double tot[NC];
+    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
+        for (int c = 0; c < NC; c++) {
+            double *sc = a.scal + c * S_COUNT;
+            if (sc[S_DONE] != 0.0) continue;
+            double nr_old = sc[S_NR];
+            double nr = tot[c];
+            sc[S_NR] = nr;
+            sc[S_BETA] = nr / nr_old;
+            bool stop = (nr < (double)a.tol * sc[S_NR0]) || (it >= a.limit);
+            sc[S_ITER] = (double)(it + 1);
+            if (stop) sc[S_DONE] = 1.0;
+        }
+}
+
+// ---------------- mixed precision CG (FP32 vectors, FP64 dots) ----------------
+struct CGArgsF {
+    const Stencil *st;
+    const int4 *nbr;
+    const double *b;    // FP64 right-hand side (for the refinement)
+    int n, nc, nb;
+    float *x[2];
+    float *d[2];
+    float *r;
+    double *partials;
+    unsigned *counter;
+    double *scal;
+    float tol;
+    int limit;
+    int refine_every;
+};
+
+struct StencilF {
+    float at, al, ab, ar, sum, a1;
+};
+
+__device__ inline StencilF to_f(const Stencil &s)
+{
+    return StencilF{(float)s.at, (float)s.al, (float)s.ab, (float)s.ar, (float)s.sum, (float)s.a1};
+}
+
+template <int NC>
+__global__ void k_cgf_init(CGArgsF a, const double *x0)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[NC];
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
+    if (k < a.n) {
+        Stencil s = a.st[k];
+        int4 nb = a.nbr[k];
+        for (int c = 0; c < NC; c++) {
+            size_t o = (size_t)c * a.n;
+            const double *xv = x0 + o;
+            double xc = xv[k];
+            double rv = a.b[o + k] - s.a1 * xc + lap(s, nb, xv, xc);   // FP64 initial residual
+            a.x[0][o + k] = (float)xc;
+            a.r[o + k] = (float)rv;
+            a.d[0][o + k] = (float)rv;
+            acc[c] = rv * rv;
+        }
+    }
+    block_reduce_store<NC>(acc, a.partials, a.nb);
+    double tot[NC];
+    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
+        for (int c = 0; c < NC; c++) {
+            double *sc = a.scal + c * S_COUNT;
+            sc[S_NR0] = tot[c];
+            sc[S_NR] = tot[c];
+            sc[S_DONE] = (tot[c] > 0) ? 0.0 : 1.0;
+            sc[S_ITER] = 0.0;
+        }
+}
+
+template <int NC>
+__global__ void k_cgf_a(CGArgsF a, int it)
+{
+    __shared__ double sdone[NC], sbeta[NC];
+    if (threadIdx.x < NC) {
+        sdone[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_DONE];
+        sbeta[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_BETA];
+    }
+    __syncthreads();
+    bool all_done = true;
+    for (int c = 0; c < NC; c++) all_done = all_done && sdone[c] != 0.0;
+    if (all_done) return;
+    const float *dp = a.d[it & 1];
+    float *dn = a.d[(it + 1) & 1];
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[NC];
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
+    if (k < a.n) {
+        StencilF s = to_f(a.st[k]);
+        int4 nb = a.nbr[k];
+        for (int c = 0; c < NC; c++) {
+            if (sdone[c] != 0.0) continue;
+            size_t o = (size_t)c * a.n;
+            const float *rv = a.r + o, *dv = dp + o;
+            float beta = (float)sbeta[c];
+            auto D = [&](int j) { return it == 0 ? dv[j] : rv[j] + beta * dv[j]; };
+            float dc = D(k);
+            float ut = nb.x >= 0 ? D(nb.x) : 0.0f;
+            float ul = nb.y >= 0 ? D(nb.y) : 0.0f;
+            float ub = nb.z >= 0 ? D(nb.z) : 0.0f;
+            float ur = nb.w >= 0 ? D(nb.w) : 0.0f;
+            float L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - dc * s.sum;
+            dn[o + k] = dc;
+            acc[c] = (double)rv[k] * (double)(s.a1 * dc - L);
+        }
+    }
+    block_reduce_store<NC>(acc, a.partials, a.nb);
+    double tot[NC];
+    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
+        for (int c = 0; c < NC; c++) {
+            double *sc = a.scal + c * S_COUNT;
+            if (sc[S_DONE] != 0.0) continue;
+            sc[S_RDOTQ] = tot[c];
+            sc[S_ALPHA] = sc[S_NR] / tot[c];
+        }
+}
+
+template <int NC>
+__global__ void k_cgf_b(CGArgsF a, int it)
+{
+    __shared__ double sdone[NC], salpha[NC];
+    if (threadIdx.x < NC) {
+        sdone[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_DONE];
+        salpha[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_ALPHA];
+    }
+    __syncthreads();
+    bool all_done = true;
+    for (int c = 0; c < NC; c++) all_done = all_done && sdone[c] != 0.0;
+    if (all_done) return;
+    const float *xp = a.x[it & 1];
+    float *xn = a.x[(it + 1) & 1];
+    const float *dcur = a.d[(it + 1) & 1];
+    const bool refine = a.refine_every > 0 && ((it + 1) % a.refine_every == 0);
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[NC];
+    for (int c = 0; c < NC; c++) acc[c] = 0.0;
+    if (k < a.n) {
+        Stencil s64 = a.st[k];
+        StencilF s = to_f(s64);
+        int4 nb = a.nbr[k];
+        for (int c = 0; c < NC; c++) {
+            if (sdone[c] != 0.0) continue;
+            size_t o = (size_t)c * a.n;
+            const float *xv = xp + o, *dv = dcur + o;
+            float alpha = (float)salpha[c];
+            auto X = [&](int j) { return xv[j] + alpha * dv[j]; };
+            float xc = X(k);
+            float ut = nb.x >= 0 ? X(nb.x) : 0.0f;
+            float ul = nb.y >= 0 ? X(nb.y) : 0.0f;
+            float ub = nb.z >= 0 ? X(nb.z) : 0.0f;
+            float ur = nb.w >= 0 ? X(nb.w) : 0.0f;
+            double rv;
+            if (refine) {   // residual replacement: b - A x evaluated in FP64
+                double L = (double)ut * s64.at + (double)ul * s64.al + (double)ub * s64.ab + (double)ur * s64.ar -
+                           (double)xc * s64.sum;
+                rv = a.b[o + k] - s64.a1 * (double)xc + L;
+            } else {
+                float L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - xc * s.sum;
+                rv = (double)((float)a.b[o + k] - s.a1 * xc + L);
+            }
+            xn[o + k] = xc;
+            a.r[o + k] = (float)rv;
+            acc[c] = rv * rv;
+        }
+    }
+    block_reduce_store<NC>(acc, a.partials, a.nb);
+    double tot[NC];
+    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
+        for (int c = 0; c < NC; c++) {
+            double *sc = a.scal + c * S_COUNT;
+            if (sc[S_DONE] != 0.0) continue;
+            double nr_old = sc[S_NR];
+            double nr = tot[c];
+            sc[S_NR] = nr;
+            sc[S_BETA] = nr / nr_old;
+            bool stop = (nr < (double)a.tol * sc[S_NR0]) || (it >= a.limit);
+            sc[S_ITER] = (double)(it + 1);
+            if (stop) sc[S_DONE] = 1.0;
+        }
+}
+
+// final colours: image = float(u0 + x_final) on hole points; sum of (prev - new)^2
+template <typename T>
+__global__ void k_poisson_write(float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0, int nc,
+                                const int *__restrict__ pts, int n, T *x0, T *x1, const double *__restrict__ scal,
+                                double *__restrict__ partials, unsigned *counter, double *result)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[1] = {0.0};
+    if (k < n) {
+        size_t p = (size_t)pts[k];
+        for (int ch = 0; ch < nc; ch++) {
+            int it = (int)scal[ch * S_COUNT + S_ITER];
+            const T *xf = (it & 1) ? x1 : x0;
+            double u0 = (double)orig[p * Cs + c0 + ch];
+            float v = (float)(u0 + (double)xf[(size_t)ch * n + k]);
+            float prev = img[p * Cs + c0 + ch];
+            float d = prev - v;
+            acc[0] += (double)(d * d);
+            img[p * Cs + c0 + ch] = v;
+        }
+    }
+    int nb = gridDim.x;
+    block_reduce_store<1>(acc, partials, nb);
+    double tot[1];
+    if (last_block_total<1>(partials, nb, counter, tot) && threadIdx.x == 0) *result = tot[0];
+}
+
+inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+double read_scalar(const double *d, hipStream_t s)
+{
+    double v = 0;
+    NLI_HIP(hipMemcpyAsync(&v, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    NLI_HIP(hipStreamSynchronize(s));
+    return v;
+}
+
+void upload_weights(UpdWork &w, int P, float sigma, hipStream_t s)
+{
+    std::vector<float> wh((size_t)P * P);
+    gaussian_weights(P, P, sigma, sigma, wh.data());
+    w.weights.ensure(wh.size());
+    NLI_HIP(hipMemcpyAsync(w.weights.p, wh.data(), wh.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    NLI_HIP(hipStreamSynchronize(s));
+}
+
+// scalar/partials/counter layout in w.scalars (doubles): [0, S_COUNT*4) scalars,
+// then one slot used as the result; counters live in w.flag (as unsigned).
+template <int NC>
+void run_cg(const CGParams &cg, int n, UpdWork &w, double *x0, hipStream_t s, CGStats *stats, const Stencil *st,
+            const int4 *nbr, int &final_parity_unused)
+{
+    (void)final_parity_unused;
+    const int nb = (int)grid_for(n);
+    unsigned *counter = (unsigned *)w.flag.p;
+    double *scal = w.scalars.p;
+    if (cg.precision == 0) {
+        CGArgs a;
+        a.st = st;
+        a.nbr = nbr;
+        a.b = w.b.p;
+        a.n = n;
+        a.nc = NC;
+        a.nb = nb;
+        a.x[0] = x0;
+        a.x[1] = w.x1.p;
+        a.d[0] = w.d0.p;
+        a.d[1] = w.d1.p;
+        a.r = w.r.p;
+        a.partials = w.partials.p;
+        a.counter = counter;
+        a.scal = scal;
+        a.tol = cg.tol;
+        a.limit = cg.limit;
+        k_cg_init<NC><<<nb, kBlock, 0, s>>>(a);
+        NLI_LAUNCH_CHECK();
+        const int poll = 8;
+        for (int it = 0; it <= cg.limit; it++) {
+            k_cg_a<NC><<<nb, kBlock, 0, s>>>(a, it);
+            k_cg_b<NC><<<nb, kBlock, 0, s>>>(a, it);
+            NLI_LAUNCH_CHECK();
+            if ((it + 1) % poll == 0 && it < cg.limit) {
+                double flags[kMaxNC * S_COUNT];
+                NLI_HIP(hipMemcpyAsync(flags, scal, sizeof(double) * NC * S_COUNT, hipMemcpyDeviceToHost, s));
+                NLI_HIP(hipStreamSynchronize(s));
+                bool all = true;
+                for (int c = 0; c < NC; c++) all = all && flags[c * S_COUNT + S_DONE] != 0.0;
+                if (all) break;
+            }
+        }
+    } else {
+        CGArgsF a;
+        a.st = st;
+        a.nbr = nbr;
+        a.b = w.b.p;
+        a.n = n;
+        a.nc = NC;
+        a.nb = nb;
+        a.x[0] = w.xs0.p;
+        a.x[1] = w.xs1.p;
+        a.d[0] = w.ds0.p;
+        a.d[1] = w.ds1.p;
+        a.r = w.rs.p;
+        a.partials = w.partials.p;
+        a.counter = counter;
+        a.scal = scal;
+        a.tol = cg.tol;
+        a.limit = cg.limit;
+        a.refine_every = cg.refine_every;
+        k_cgf_init<NC><<<nb, kBlock, 0, s>>>(a, x0);
+        NLI_LAUNCH_CHECK();
+        const int poll = 8;
+        for (int it = 0; it <= cg.limit; it++) {
+            k_cgf_a<NC><<<nb, kBlock, 0, s>>>(a, it);
+            k_cgf_b<NC><<<nb, kBlock, 0, s>>>(a, it);
+            NLI_LAUNCH_CHECK();
+            if ((it + 1) % poll == 0 && it < cg.limit) {
+                double flags[kMaxNC * S_COUNT];
+                NLI_HIP(hipMemcpyAsync(flags, scal, sizeof(double) * NC * S_COUNT, hipMemcpyDeviceToHost, s));
+                NLI_HIP(hipStreamSynchronize(s));
+                bool all = true;
+                for (int c = 0; c < NC; c++) all = all && flags[c * S_COUNT + S_DONE] != 0.0;
+                if (all) break;
+            }
+        }
+    }
+    if (stats) {
+        double h[kMaxNC * S_COUNT];
+        NLI_HIP(hipMemcpyAsync(h, scal, sizeof(double) * NC * S_COUNT, hipMemcpyDeviceToHost, s));
+        NLI_HIP(hipStreamSynchronize(s));
+        for (int c = 0; c < NC; c++) {
+            stats->iterations[c] = (int)h[c * S_COUNT + S_ITER];
+            stats->nr0[c] = h[c * S_COUNT + S_NR0];
+            stats->nr[c] = h[c * S_COUNT + S_NR];
+        }
+    }
+}
+
+double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *image, const float *orig, int Cs,
+               int c0, int nc, const int *hole_pts, int n, const uint8_t *ext, const int32_t *nnf,
+               const float *conf, int W, int H, UpdWork &w, CGStats *stats, hipStream_t s)
+{
+    const size_t np = (size_t)W * H;
+    upload_weights(w, patch, sigma, s);
+    w.grad.ensure(np * Cs * 2);
+    gradient(image, W, H, Cs, w.grad.p, s);
+    w.a1.ensure(np);
+    w.a2.ensure(np);
+    w.f1.ensure(np * nc);
+    w.F1.ensure(np * nc);
+    w.F2.ensure(np * nc);
+    w.f2.ensure(np * nc);
+    dim3 g2((W + kBlock - 1) / kBlock, H);
+    k_poisson_coef<<<g2, kBlock, 0, s>>>(image, w.grad.p, Cs, c0, nc, ext, (const int2 *)nnf, conf,
+                                         w.weights.p, patch, lambda, W, H, w.a1.p, w.a2.p, w.f1.p, w.F1.p, w.F2.p);
+    NLI_LAUNCH_CHECK();
+    k_divergence<<<g2, kBlock, 0, s>>>(ext, w.F1.p, w.F2.p, nc, W, H, lambda != 1 ? 1 : 0, w.f2.p);
+    NLI_LAUNCH_CHECK();
+    const int nb = (int)grid_for(std::max(n, 1));
+    w.scalars.ensure(kMaxNC * S_COUNT + 8);
+    w.partials.ensure((size_t)kMaxNC * nb + 8);
+    w.flag.ensure(4);
+    NLI_HIP(hipMemsetAsync(w.flag.p, 0, 4 * sizeof(int), s));
+    double *result = w.scalars.p + kMaxNC * S_COUNT;
+    if (n == 0) {
+        if (stats)
+            for (int c = 0; c < nc; c++) { stats->iterations[c] = 0; stats->nr0[c] = 0; stats->nr[c] = 0; }
+        return 0.0;
+    }
+    w.lidx.ensure(np);
+    fill_i32(w.lidx.p, -1, np, s);
+    k_lidx<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p);
+    NLI_LAUNCH_CHECK();
+    w.stencil.ensure((size_t)n * (sizeof(Stencil) / sizeof(double)));
+    w.nbr.ensure((size_t)n * 4);
+    Stencil *st = (Stencil *)w.stencil.p;
+    int4 *nbr = (int4 *)w.nbr.p;
+    k_cg_setup<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p, w.a1.p, w.a2.p, W, H, st, nbr);
+    NLI_LAUNCH_CHECK();
+    w.b.ensure((size_t)n * nc);
+    w.x0.ensure((size_t)n * nc);
+    k_cg_rhs<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, image, orig, Cs, c0, nc, w.a1.p, w.a2.p, w.f1.p, w.f2.p, W,
+                                            H, w.b.p, w.x0.p);
+    NLI_LAUNCH_CHECK();
+    if (cg.precision == 0) {
+        w.x1.ensure((size_t)n * nc);
+        w.d0.ensure((size_t)n * nc);
+        w.d1.ensure((size_t)n * nc);
+        w.r.ensure((size_t)n * nc);
+    } else {
+        w.xs0.ensure((size_t)n * nc);
+        w.xs1.ensure((size_t)n * nc);
+        w.ds0.ensure((size_t)n * nc);
+        w.ds1.ensure((size_t)n * nc);
+        w.rs.ensure((size_t)n * nc);
+    }
+    int unused = 0;
+    switch (nc) {
+    case 1: run_cg<1>(cg, n, w, w.x0.p, s, stats, st, nbr, unused); break;
+    case 2: run_cg<2>(cg, n, w, w.x0.p, s, stats, st, nbr, unused); break;
+    case 3: run_cg<3>(cg, n, w, w.x0.p, s, stats, st, nbr, unused); break;
+    case 4: run_cg<4>(cg, n, w, w.x0.p, s, stats, st, nbr, unused); break;
+    default: throw std::invalid_argument("poisson: 1 to 4 channels");
+    }
+    unsigned *counter = (unsigned *)w.flag.p + 1;
+    if (cg.precision == 0)
+        k_poisson_write<double><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.x0.p, w.x1.p,
+                                                      w.scalars.p, w.partials.p, counter, result);
+    else
+        k_poisson_write<float><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.xs0.p, w.xs1.p,
+                                                     w.scalars.p, w.partials.p, counter, result);
+    NLI_LAUNCH_CHECK();
+    return read_scalar(result, s);
+}
+
+}  // namespace
+
+double update(int kind, int patch, float sigma, float lambda, const CGParams &cg, float *image, const float *orig,
+              int Cs, int c0, int nc, const uint8_t *hole, const int *hole_pts, int n_hole, const uint8_t *ext,
+              const int32_t *nnf, const float *conf, int W, int H, UpdWork &w, CGStats *stats, hipStream_t s)
+{
+    if (patch < 1 || patch > kMaxP) throw std::invalid_argument("patch side must be in [1, 15]");
+    if (nc < 1 || nc > kMaxNC || c0 < 0 || c0 + nc > Cs) throw std::invalid_argument("bad channel range");
+    if (kind == UPD_POISSON)
+        return poisson(patch, sigma, lambda, cg, image, orig, Cs, c0, nc, hole_pts, n_hole, ext, nnf, conf, W, H, w,
+                       stats, s);
+    if (kind == UPD_MEANS || kind == UPD_MEDIANS) {
+        upload_weights(w, patch, sigma, s);
+        const int n = n_hole;
+        if (n == 0) return 0.0;
+        w.x1.ensure(((size_t)n * nc + 1) / 2 + 1);   // new values (floats) live in x1's storage
+        float *newv = (float *)w.x1.p;
+        w.pflag.ensure(n);
+        if (kind == UPD_MEANS) {
+            k_nlmeans<<<grid_for(n), kBlock, 0, s>>>(image, Cs, c0, nc, hole_pts, n, hole, (const int2 *)nnf, conf,
+                                                     w.weights.p, patch, W, H, newv, w.pflag.p);
+        } else {
+            // two LDS columns of P*P floats per thread; 64 threads up to P = 11, 32 above
+            const int bs = (patch <= 11) ? 64 : 32;
+            const size_t lds = ((size_t)patch * patch * (2 * bs + 1)) * sizeof(float);
+            k_nlmedians<<<(n + bs - 1) / bs, bs, lds, s>>>(image, Cs, c0, nc, hole_pts, n, hole, (const int2 *)nnf,
+                                                           conf, w.weights.p, patch, W, H, newv, w.pflag.p);
+        }
+        NLI_LAUNCH_CHECK();
+        const int nb = (int)grid_for(n);
+        w.scalars.ensure(kMaxNC * S_COUNT + 8);
+        w.partials.ensure((size_t)kMaxNC * nb + 8);
+        w.flag.ensure(4);
+        NLI_HIP(hipMemsetAsync(w.flag.p, 0, 4 * sizeof(int), s));
+        double *result = w.scalars.p + kMaxNC * S_COUNT;
+        k_write_back<<<nb, kBlock, 0, s>>>(image, Cs, c0, nc, hole_pts, n, newv, w.pflag.p, w.partials.p,
+                                           (unsigned *)w.flag.p + 1, result);
+        NLI_LAUNCH_CHECK();
+        return read_scalar(result, s);
+    }
+    throw std::invalid_argument("unknown image updating kind");
+}
+
+}  // namespace nli

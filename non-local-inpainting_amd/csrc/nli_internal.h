@@ -1,0 +1,237 @@
+// nli_internal.h -- shared declarations of libnli (device buffers, launch
+// wrappers).  Everything here operates on device pointers and a HIP stream;
+// the C-ABI (nli_capi.hip) and the device-resident driver are built on it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace nli {
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const std::string &what) : std::runtime_error(what), code(c) {}
+};
+
+#define NLI_HIP(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::nli::HipError(e_, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+#define NLI_LAUNCH_CHECK() NLI_HIP(hipGetLastError())
+
+// Grow-only device buffer owned by a context.
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n)
+    {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DBuf &operator=(DBuf &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    T *ensure(size_t count)
+    {
+        if (count == 0) count = 1;
+        if (count > n) {
+            release();
+            NLI_HIP(hipMalloc(&p, count * sizeof(T)));
+            n = count;
+        }
+        return p;
+    }
+};
+
+// Distance kinds and updater kinds; numbering shared with the public header.
+enum DistKind { DIST_L2 = 0, DIST_L1 = 1, DIST_L2COMBINED = 2, DIST_L2RGBD = 3 };
+enum UpdKind { UPD_MEANS = 0, UPD_MEDIANS = 1, UPD_POISSON = 2, UPD_DEPTH = 3 };
+
+struct DistDesc {
+    int kind;
+    int patch;
+    float sigma;
+    float lambda;
+    float lambda_d;
+};
+
+// ---- patch weights / filters (host side, double libm as the reference) ----
+void gaussian_weights(int size_x, int size_y, float sigma_x, float sigma_y, float *out);
+float subsampling_rate(float ratio, int scales);
+int sample_size(int size, float factor);
+// coarsest-level Poisson fill of the masked pixels (host, in place, HWC)
+void poisson_fill(float *img, const uint8_t *mask, int W, int H, int C);
+
+// ---- generic device helpers (k_masks.hip) ----
+// Order-preserving compaction: writes the indices i < n with flag[i] != 0 to out,
+// returns the count (synchronises the stream to read it).
+int compact_indices(const uint8_t *flag, int n, int *out, int *scratch_counts, int *d_total,
+                    hipStream_t s);
+void fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
+void fill_i32(int32_t *p, int32_t v, size_t n, hipStream_t s);
+void fill_f32(float *p, float v, size_t n, hipStream_t s);
+void fill_f64(double *p, double v, size_t n, hipStream_t s);
+
+// ---- gradient (k_patchmatch.hip) ----
+void gradient(const float *img, int W, int H, int C, float *grad, hipStream_t s);
+
+// ---- masks (k_masks.hip) ----
+void extended_domain(const uint8_t *hole, int W, int H, int P, uint8_t *ext, uint8_t *scratch,
+                     hipStream_t s);
+// target/source masks of a scale; erode = the right/bottom erosion of inpaint_internal.
+// *d_nonempty (device int) is set to 1 if the eroded source (before the margin) is non-empty.
+void scale_masks(const uint8_t *ext, int W, int H, int margin, int erode, uint8_t *target,
+                 uint8_t *source, int *d_nonempty, hipStream_t s);
+void scale_nnf(const int32_t *lower_nnf, const int *lower_hole_pts, int n_lower, int lW, int lH,
+               int uW, int uH, int32_t *upper_nnf, hipStream_t s);
+void copy_into_hole(const float *src, float *dst, const int *hole_pts, int n, int C, hipStream_t s);
+void distance_transform(const uint8_t *mask, int W, int H, float *out, float *g_scratch,
+                        int *st_scratch, hipStream_t s);
+void confidence(const uint8_t *mask, const float *dt, int W, int H, float decay, float asym,
+                float *out, hipStream_t s);
+
+// ---- pyramid (k_pyramid.hip) ----
+void downsample_with_mask(const float *img, const uint8_t *mask, int W, int H, int C, float factor,
+                          float threshold, int mask_val, float *out_img, uint8_t *out_mask,
+                          float *scratch, hipStream_t s);
+void downsample(const float *img, int W, int H, int C, float factor, float *out, float *scratch,
+                hipStream_t s);
+void downsample_mask(const uint8_t *mask, int W, int H, float factor, float threshold,
+                     uint8_t *out, float *scratch, hipStream_t s);
+void upsample(const float *in, int W, int H, int C, int W2, int H2, float *out, hipStream_t s);
+
+// ---- PatchMatch (k_patchmatch.hip) ----
+struct PMConfig {
+    int iterations;
+    int shots;
+    int window;
+    int first_step;   // jump-flood step of the first sweep; 0 = automatic
+};
+
+struct PMWork {          // per-call scratch, grown on demand
+    DBuf<float> sgrad, tgrad, weights, dist0, dist1;
+    DBuf<int32_t> nnf1;
+    DBuf<int> tiles, counts, total;
+    DBuf<uint8_t> tflags;
+};
+
+// Full PatchMatch::calculate on device.  init_nnf may be null (random init).
+// out_nnf receives the field (W x H x 2, (-1,-1) outside the targets); out_dist (optional)
+// the distance of every target to its entry.
+void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const uint8_t *smask,
+                int sW, int sH, const float *tgt, const uint8_t *tmask, int tW, int tH, int C,
+                const int32_t *init_nnf, uint64_t seed, int32_t *out_nnf, float *out_dist,
+                PMWork &w, hipStream_t s);
+
+// Distances of every target to a fixed NNF (FLT_MAX elsewhere).
+void patch_distance_field(const DistDesc &d, const float *src, int sW, int sH, const float *tgt,
+                          int tW, int tH, int C, const int32_t *nnf, const uint8_t *tmask,
+                          float *out, PMWork &w, hipStream_t s);
+
+// ---- image updating (k_update.hip) ----
+struct UpdWork {
+    DBuf<float> weights, grad, conf_one;
+    DBuf<double> a1, a2, f1, f2, F1, F2, partials, scalars;
+    DBuf<double> x0, x1, r, d0, d1, b, u0, stencil;
+    DBuf<float> xs0, xs1, rs, ds0, ds1, bs;   // mixed-precision CG vectors
+    DBuf<int> pts, nbr, lidx, counts, total, flag;
+    DBuf<uint8_t> pflag;
+};
+
+struct CGParams {
+    float tol;
+    int limit;
+    int precision;       // 0 = FP64 (reference recurrence), 1 = FP32 vectors + FP64 reductions
+    int refine_every;    // mixed precision: recompute r = b - A x in FP64 every k iterations
+};
+
+struct CGStats {
+    int iterations[4];
+    double nr0[4];
+    double nr[4];
+};
+
+// AImageUpdating::update for the four updaters, on device.  image (W x H x Cs, channel
+// stride Cs) is updated in place on channels [c0, c0+nc); orig is u0 (same stride).
+// hole_pts (n_hole, row-major) are the inpainting-domain points.  Returns the total
+// difference (synchronises the stream).
+double update(int kind, int patch, float sigma, float lambda, const CGParams &cg, float *image,
+              const float *orig, int Cs, int c0, int nc, const uint8_t *hole, const int *hole_pts,
+              int n_hole, const uint8_t *ext, const int32_t *nnf, const float *conf, int W, int H,
+              UpdWork &w, CGStats *stats, hipStream_t s);
+
+// ---- the multiscale driver (nli_driver.cpp) ----
+enum InitKind { INIT_BLACK = 0, INIT_AVG = 1, INIT_NONE = 2, INIT_POISSON = 3 };
+
+struct EmptySource : std::runtime_error {
+    EmptySource()
+        : std::runtime_error("ERROR: Empty source mask (no complete patches to copy from. This may happen due to "
+                             "a too big inpainting domain, too big patch, or too much downscaling)")
+    {
+    }
+};
+
+struct InpaintConfig {
+    int method, patch;
+    float sigma, lambda, lambda_rgb, lambda_d;
+    int iters;
+    float tol;
+    int scales;
+    float coarse, conf_t, conf_a;
+    int init, pm_iters, pm_shots, pm_window;
+    float cg_tol;
+    int cg_limit, cg_precision, cg_refine_every;
+    uint64_t seed;
+};
+
+struct InpaintReport {
+    int scales;
+    int iterations[32];
+    int width[32], height[32];
+    int hole_pixels[32], target_pixels[32];
+    double last_total[32];
+    int cg_iterations_total;
+    int pm_calls;
+    double seconds_total, seconds_patchmatch, seconds_update;
+};
+
+struct DriverWork {
+    std::vector<DBuf<float>> img;
+    std::vector<DBuf<uint8_t>> mask;
+    DBuf<float> scratch, conf, dtg, up, orig;
+    DBuf<int> dtst, hole_pts, low_pts, counts, total, nonempty;
+    DBuf<uint8_t> ext, tgt, src, bscr;
+    DBuf<int32_t> nnfA, nnfB, nnfC;
+};
+
+void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask, int W, int H, int C,
+             float *d_out, InpaintReport &rep, DriverWork &dw, PMWork &pmw, UpdWork &uw, hipStream_t s);
+
+}  // namespace nli

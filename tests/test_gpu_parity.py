@@ -1,0 +1,287 @@
+"""GPU parity: libnli.so kernels (through the C-ABI) against the CPU oracle on the
+same seeded inputs.
+
+Bars: bit-exact for the gradient, the distance transform, the pyramid (images and
+masks), every mask, the patch distances at a fixed NNF and NL-means; NL-medians
+equal except where equal values tie at the median threshold; NL-Poisson/Depth and
+the CG solution within 1e-5 (absolute, images in [0, 1]; relative for depth);
+confidence within one float ulp (the exponential is a different libm).
+PatchMatch is randomised with a different search schedule, so it is checked by
+invariants and by its NNF energy against the serial reference run (GPU energy at
+most 1 % above it).
+"""
+import numpy as np
+import pytest
+
+from conftest import bitexact, golden
+
+pytestmark = pytest.mark.gpu
+
+
+def ulps(a, b):
+    a = np.asarray(a, np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, np.float32).view(np.int32).astype(np.int64)
+    return np.abs(a - b)
+
+
+# ---------------------------------------------------------------- numerics
+@pytest.mark.parametrize("W,H,C", [(37, 29, 3), (64, 48, 1), (33, 65, 4), (256, 200, 3)])
+def test_gradient(ctx, oracle, syn, W, H, C):
+    img = syn.texture(W, H, 3, 3, W)[..., :1] if C == 1 else syn.texture(W, H, 3, 3, W)
+    if C == 4:
+        img = np.concatenate([img, syn.depth_planes(W, H, 5, 1)[..., None]], axis=2)
+    assert bitexact(ctx.gradient(img), oracle.gradient(img))
+
+
+@pytest.mark.parametrize("W,H,seed", [(37, 29, 0), (128, 96, 1), (301, 207, 2)])
+def test_distance_transform_and_confidence(ctx, oracle, syn, W, H, seed):
+    m = syn.ellipse_holes(W, H, 4, 0.25, seed, border=1)
+    assert bitexact(ctx.distance_transform(m), oracle.distance_transform(m))
+    c_gpu, c_cpu = ctx.confidence(m, 5.0, 0.1), oracle.confidence(m, 5.0, 0.1)
+    assert ulps(c_gpu, c_cpu).max() <= 1
+    assert bitexact(ctx.confidence(m, 0.0, 0.3), oracle.confidence(m, 0.0, 0.3))
+
+
+def test_golden_numerics(ctx):
+    g = golden("numerics")
+    img, hole = g["img"], g["hole"]
+    rate = float(g["rate_0p1_11"])
+    assert bitexact(ctx.gradient(img), g["gradient"])
+    assert bitexact(ctx.distance_transform(hole), g["dt"])
+    oi, om = ctx.downsample_with_mask(img, hole, rate)
+    assert bitexact(oi, g["ds_img"]) and bitexact(om, g["ds_mask"])
+    assert bitexact(ctx.downsample(img, rate), g["ds_plain"])
+    assert bitexact(ctx.downsample_mask(hole, rate), g["ds_maskonly"])
+    assert bitexact(ctx.upsample(img, 50, 41), g["up"])
+    assert bitexact(ctx.extended_domain(hole, 9), g["ext9"])
+
+
+def test_pyramid_chain_1024(ctx, oracle, syn):
+    """The 11-level pyramid of the 1024x1024 workload, level by level."""
+    img = syn.texture(1024, 1024, 3, 8, 1)
+    hole = syn.ellipse_holes(1024, 1024, 6, 0.15, 1)
+    rate = oracle.subsampling_rate(0.1, 11)
+    for level in range(10):
+        a = ctx.downsample_with_mask(img, hole, rate)
+        b = oracle.downsample_with_mask(img, hole, rate)
+        assert bitexact(a[0], b[0]) and bitexact(a[1], b[1]), level
+        up_g = ctx.upsample(b[0], img.shape[1], img.shape[0])
+        assert bitexact(up_g, oracle.upsample(b[0], img.shape[1], img.shape[0])), level
+        img, hole = b
+
+
+@pytest.mark.parametrize("factor", [0.5, 0.7197, 0.83])
+def test_plain_downsampling(ctx, oracle, syn, factor):
+    img = syn.texture(131, 97, 3, 3, 3)
+    m = syn.ellipse_holes(131, 97, 3, 0.3, 3)
+    assert bitexact(ctx.downsample(img, factor), oracle.downsample(img, factor))
+    assert bitexact(ctx.downsample_mask(m, factor), oracle.downsample_mask(m, factor))
+    a = ctx.downsample_with_mask(img, m, factor, 0.4, True)
+    b = oracle.downsample_with_mask(img, m, factor, 0.4, True)
+    assert bitexact(a[0], b[0]) and bitexact(a[1], b[1])
+
+
+# ---------------------------------------------------------------- masks
+@pytest.mark.parametrize("P", [3, 7, 9, 11])
+def test_masks(ctx, oracle, syn, P):
+    hole = syn.stripe_holes(160, 120, 3, 9, P)       # stripes touch the image border
+    assert bitexact(ctx.extended_domain(hole, P), oracle.extended_domain(hole, P))
+    for erode in (False, True):
+        e, t, s = ctx.scale_masks(hole, P, erode)
+        e2, t2, s2, empty = oracle.scale_masks(hole, P, erode)
+        assert bitexact(e, e2) and bitexact(t, t2) and bitexact(s, s2)
+
+
+def test_empty_source_is_reported(ctx, nli):
+    hole = np.ones((40, 40), np.uint8)
+    with pytest.raises(nli.EmptySourceError):
+        ctx.scale_masks(hole, 7, True)
+
+
+# ---------------------------------------------------------------- distances
+def test_patch_distance_golden(ctx):
+    g = golden("field64x48")
+    img, img4, nnf, tgt = g["img"], g["img4"], g["nnf"], g["tgt"]
+    for kind in ["l2", "l1", "l2combined"]:
+        assert bitexact(ctx.patch_distance(kind, img, img, nnf, tgt, 9, 10000.0, 0.3, 0.7), g[f"dist_{kind}"]), kind
+    assert bitexact(ctx.patch_distance("l2rgbd", img4, img4, nnf, tgt, 9, 10000.0, 0.3, 0.7), g["dist_l2rgbd"])
+
+
+@pytest.mark.parametrize("kind", ["l2", "l1", "l2combined", "l2rgbd"])
+@pytest.mark.parametrize("C", [1, 2, 3, 4])
+def test_patch_distance_random_fields(ctx, oracle, syn, kind, C):
+    if kind == "l2rgbd" and C != 4:
+        pytest.skip("RGBD needs 4 channels")
+    W, H, P, r = 90, 70, 7, 3
+    img = syn.texture(W, H, 3, 3, C)
+    if C == 4:
+        img = np.concatenate([img, syn.depth_planes(W, H, 5, C)[..., None]], axis=2)
+    img = np.ascontiguousarray(img[..., :C])
+    rng = np.random.default_rng(C)
+    tm = np.zeros((H, W), np.uint8)
+    tm[r:H - r, r:W - r] = rng.integers(0, 2, (H - 2 * r, W - 2 * r))
+    nnf = np.stack([rng.integers(r, W - r, (H, W)), rng.integers(r, H - r, (H, W))], axis=2).astype(np.int32)
+    dp = oracle.dist_params(kind, P, 2.5, 0.3, 0.6)
+    got = ctx.patch_distance(kind, img, img, nnf, tm, P, 2.5, 0.3, 0.6)
+    assert bitexact(got, oracle.patch_distance_field(dp, img, img, nnf, tm))
+
+
+def test_rgbd_needs_four_channels(ctx, syn):
+    img = syn.texture(40, 40, 3, 3, 1)
+    nnf = np.zeros((40, 40, 2), np.int32)
+    with pytest.raises(ValueError, match="Channels do not match"):
+        ctx.patch_distance("l2rgbd", img, img, nnf, np.zeros((40, 40), np.uint8))
+
+
+# ---------------------------------------------------------------- PatchMatch
+def energy(oracle, dp, img, nnf, tgt):
+    d = oracle.patch_distance_field(dp, img, img, nnf, tgt)
+    sel = tgt.astype(bool)
+    assert np.all(nnf[sel][:, 0] >= 0)
+    return float(np.sum(d[sel].astype(np.float64)))
+
+
+@pytest.mark.parametrize("kind,lam", [("l2combined", 0.05), ("l2", 0.0), ("l1", 0.0)])
+def test_patchmatch_energy_vs_serial_reference(ctx, oracle, syn, kind, lam):
+    W = H = 160
+    img = syn.texture(W, H, 3, 8, 1)
+    hole = syn.ellipse_holes(W, H, 6, 0.15, 1)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 9)
+    dp = oracle.dist_params(kind, 9, 10000.0, lam)
+    e_gpu, e_cpu = [], []
+    for seed in (1, 2, 3):
+        n_gpu, d_gpu = ctx.patchmatch(kind, img, src, img, tgt, None, seed=seed, lam=lam, with_dist=True)
+        n_cpu = oracle.patchmatch(dp, img, src, img, tgt, None, seed=seed)
+        sel = tgt.astype(bool)
+        assert np.all(n_gpu[~sel] == -1)
+        assert np.all(src[n_gpu[sel][:, 1], n_gpu[sel][:, 0]] == 1)       # entries are source points
+        # stored distances are the reference's distances of the entries, bit for bit
+        assert bitexact(d_gpu[sel], oracle.patch_distance_field(dp, img, img, n_gpu, tgt)[sel])
+        e_gpu.append(energy(oracle, dp, img, n_gpu, tgt))
+        e_cpu.append(energy(oracle, dp, img, n_cpu, tgt))
+    ratio = np.mean(e_gpu) / np.mean(e_cpu)
+    print(f"{kind}: energy GPU/serial reference = {ratio:.4f}")
+    assert ratio <= 1.01
+
+
+def test_patchmatch_given_field_only_improves(ctx, oracle, syn):
+    W = H = 96
+    img = syn.texture(W, H, 3, 5, 4)
+    hole = syn.ellipse_holes(W, H, 3, 0.15, 4)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 7)
+    dp = oracle.dist_params("l2combined", 7, 10000.0, 0.05)
+    init = oracle.patchmatch(dp, img, src, img, tgt, None, seed=5, iters=2)
+    d0 = oracle.patch_distance_field(dp, img, img, init, tgt)
+    n, d = ctx.patchmatch("l2combined", img, src, img, tgt, init, seed=3, patch=7, lam=0.05, with_dist=True)
+    sel = tgt.astype(bool)
+    assert np.all(d[sel] <= d0[sel])
+    # zero sweeps keep every valid initial entry and its distance
+    n0, dd0 = ctx.patchmatch("l2combined", img, src, img, tgt, init, seed=3, patch=7, lam=0.05, iters=0,
+                             with_dist=True)
+    assert np.array_equal(n0, init) and bitexact(dd0[sel], d0[sel])
+
+
+def test_patchmatch_rgbd_and_invalid_init(ctx, oracle, syn):
+    W, H = 120, 90
+    img = np.concatenate([syn.texture(W, H, 3, 3, 7), syn.depth_planes(W, H, 5, 7)[..., None]], axis=2)
+    hole = syn.stripe_holes(W, H, 2, 8, 7)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 9)
+    bad = np.full((H, W, 2), -1, np.int32)         # every entry invalid: random re-initialisation
+    n = ctx.patchmatch("l2rgbd", img, src, img, tgt, bad, seed=1, lam=1.0, lam_d=0.0)
+    sel = tgt.astype(bool)
+    assert np.all(src[n[sel][:, 1], n[sel][:, 0]] == 1)
+
+
+# ---------------------------------------------------------------- updates
+def field_case(oracle, syn, W=96, H=80, P=9, seed=1, kind="l2combined"):
+    img = syn.texture(W, H, 3, 3, seed)
+    hole = syn.ellipse_holes(W, H, 4, 0.15, seed, border=6)
+    ext, tgt, src, _ = oracle.scale_masks(hole, P)
+    nnf = oracle.patchmatch(oracle.dist_params(kind, P, 10000.0, 0.05), img, src, img, tgt, None, seed=seed)
+    conf = oracle.confidence(hole, 5.0, 0.1)
+    return img, hole, ext, nnf, conf
+
+
+def test_nlmeans_bitexact(ctx, oracle, syn):
+    img, hole, ext, nnf, conf = field_case(oracle, syn)
+    orig = img.copy()
+    img = img + np.float32(0.01)                       # image differs from orig inside the scale loop
+    a, ta = ctx.update("nlmeans", img, orig, hole, ext, nnf, conf)
+    b, tb = oracle.update("nlmeans", img, orig, hole, ext, nnf, conf)
+    assert bitexact(a, b)
+    assert abs(ta - tb) <= 1e-12 * max(1.0, tb)
+    a, _ = ctx.update("nlmeans", img, orig, hole, ext, nnf, None)     # empty confidence
+    b, _ = oracle.update("nlmeans", img, orig, hole, ext, nnf, None)
+    assert bitexact(a, b)
+
+
+def test_nlmedians(ctx, oracle, syn):
+    img, hole, ext, nnf, conf = field_case(oracle, syn, kind="l1")
+    a, ta = ctx.update("nlmedians", img, img, hole, ext, nnf, conf)
+    b, tb = oracle.update("nlmedians", img, img, hole, ext, nnf, conf)
+    sel = hole.astype(bool)
+    same = np.all(a[sel] == b[sel], axis=1)
+    print(f"nlmedians: {same.mean() * 100:.3f} % of hole pixels identical")
+    assert same.mean() >= 0.999
+    assert abs(ta - tb) <= 1e-3 * tb
+    rng = np.random.default_rng(3)                      # a random field has no value ties
+    sp = np.argwhere(oracle.scale_masks(hole, 9)[2])
+    rnd = np.full_like(nnf, -1)
+    t = np.argwhere(oracle.scale_masks(hole, 9)[1])
+    pick = sp[rng.integers(0, len(sp), len(t))]
+    rnd[t[:, 0], t[:, 1], 0] = pick[:, 1]
+    rnd[t[:, 0], t[:, 1], 1] = pick[:, 0]
+    a, _ = ctx.update("nlmedians", img, img, hole, ext, rnd, conf, patch=7)
+    b, _ = oracle.update("nlmedians", img, img, hole, ext, rnd, conf, patch=7)
+    assert bitexact(a, b)
+
+
+@pytest.mark.parametrize("lam,tol,limit", [(0.05, 1e-4, 100), (0.5, 1e-6, 1000), (1.0, 1e-6, 50), (0.0, 1e-6, 300)])
+def test_nlpoisson(ctx, oracle, syn, lam, tol, limit):
+    img, hole, ext, nnf, conf = field_case(oracle, syn)
+    orig = img.copy()
+    a, ta, st = ctx.update("nlpoisson", img, orig, hole, ext, nnf, conf, lam=lam, cg_tol=tol, cg_limit=limit,
+                           stats=True)
+    b, tb, so = oracle.update("nlpoisson", img, orig, hole, ext, nnf, conf, lam=lam, cg_tol=tol, cg_limit=limit,
+                              stats=True)
+    err = np.max(np.abs(a.astype(np.float64) - b))
+    print(f"lambda {lam}: CG iterations GPU {list(st.cg_iterations[:3])} oracle {list(so.cg_iterations[:3])}, "
+          f"max |diff| {err:.2e}")
+    assert err <= 1e-5
+    for c in range(3):
+        assert abs(st.cg_iterations[c] - so.cg_iterations[c]) <= 1
+        assert abs(st.cg_nr0[c] - so.cg_nr0[c]) <= 1e-9 * so.cg_nr0[c]
+    assert abs(ta - tb) <= 1e-4 * max(tb, 1e-12)
+
+
+def test_nlpoisson_golden(ctx):
+    g = golden("field64x48")
+    a, ta = ctx.update("nlpoisson", g["img"], g["img"], g["hole"], g["ext"], g["pm_l2c_seed42"], g["conf"],
+                       lam=0.05, cg_tol=1e-4, cg_limit=100)
+    assert np.max(np.abs(a - g["upd_nlpoisson"])) <= 1e-5
+    a, _ = ctx.update("nlmeans", g["img"], g["img"], g["hole"], g["ext"], g["pm_l2c_seed42"], g["conf"])
+    assert bitexact(a, g["upd_nlmeans"])
+
+
+def test_nldepth(ctx, oracle, syn):
+    W, H = 96, 80
+    img = np.concatenate([syn.texture(W, H, 3, 3, 5), syn.depth_planes(W, H, 5, 5)[..., None]], axis=2)
+    hole = syn.stripe_holes(W, H, 2, 6, 5)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 9)
+    nnf = oracle.patchmatch(oracle.dist_params("l2rgbd", 9, 10000.0, 1.0, 0.0), img, src, img, tgt, None, seed=2)
+    conf = oracle.confidence(hole, 5.0, 0.1)
+    a, ta = ctx.update("nldepth", img, img, hole, ext, nnf, conf, lam=1.0, lam_d=0.0)
+    b, tb = oracle.update("nldepth", img, img, hole, ext, nnf, conf, lam=1.0, lam_d=0.0)
+    assert np.max(np.abs(a[..., :3] - b[..., :3])) <= 1e-5
+    assert np.max(np.abs(a[..., 3] - b[..., 3])) <= 1e-5 * np.max(np.abs(b[..., 3]))
+
+
+def test_mixed_precision_cg(ctx, oracle, syn):
+    """FP32 CG with FP64 dots and periodic FP64 residual replacement, against the FP64 oracle."""
+    img, hole, ext, nnf, conf = field_case(oracle, syn, W=128, H=112)
+    b, tb = oracle.update("nlpoisson", img, img, hole, ext, nnf, conf, lam=0.05, cg_tol=1e-8, cg_limit=400)
+    a, ta, st = ctx.update("nlpoisson", img, img, hole, ext, nnf, conf, lam=0.05, cg_tol=1e-8, cg_limit=400,
+                           cg_precision=1, cg_refine_every=8, stats=True)
+    err = np.max(np.abs(a.astype(np.float64) - b))
+    print(f"mixed precision: max |x - x_fp64| = {err:.2e}, iterations {list(st.cg_iterations[:3])}")
+    assert err <= 2e-4

@@ -1,0 +1,493 @@
+"""Pins the CPU oracle.
+
+1. Bit-exact against the golden vectors the reference's own sources produced
+   (tests/golden/make_golden.py).
+2. Live against the reference core built in place, on inputs the fixtures do
+   not cover (skipped where oracle/_ref is absent).
+3. Against what the mathematics fixes, so a plausible restatement mistake (a
+   dropped term, a transposed index, a wrong sign) fails even where the
+   reference itself is not available: brute-force EDT, set definitions of the
+   masks, FP64 brute-force patch sums, direct sparse solves of the CG system,
+   the definition of the weighted median, PatchMatch invariants.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from conftest import bitexact, golden
+
+import oracle as O
+
+
+# ---------------------------------------------------------------- golden vectors
+def test_gaussian_weights_golden(oracle):
+    g = golden("gaussian_weights")
+    for key in g.files:
+        _, size, s = key.split("_")
+        sx, sy = map(int, size.split("x"))
+        s = float(s)
+        assert bitexact(oracle.gaussian_weights(sx, sy, s, s if sy > 1 else 0.0), g[key]), key
+
+
+def test_numerics_golden(oracle):
+    g = golden("numerics")
+    img, hole = g["img"], g["hole"]
+    assert bitexact(oracle.gradient(img), g["gradient"])
+    assert bitexact(oracle.distance_transform(hole), g["dt"])
+    assert bitexact(oracle.confidence(hole, 5.0, 0.1), g["confidence"])
+    rate = oracle.subsampling_rate(0.1, 11)
+    assert np.float32(rate) == g["rate_0p1_11"]
+    oi, om = oracle.downsample_with_mask(img, hole, rate)
+    assert bitexact(oi, g["ds_img"]) and bitexact(om, g["ds_mask"])
+    oi, om = oracle.downsample_with_mask(img, hole, 0.55)
+    assert bitexact(oi, g["ds_img2"]) and bitexact(om, g["ds_mask2"])
+    assert bitexact(oracle.downsample(img, rate), g["ds_plain"])
+    assert bitexact(oracle.downsample_mask(hole, rate), g["ds_maskonly"])
+    assert bitexact(oracle.upsample(img, 50, 41), g["up"])
+    assert bitexact(oracle.extended_domain(hole, 7), g["ext7"])
+    assert bitexact(oracle.extended_domain(hole, 9), g["ext9"])
+
+
+def test_field_golden(oracle):
+    g = golden("field64x48")
+    img, img4, hole, nnf, tgt, src = g["img"], g["img4"], g["hole"], g["nnf"], g["tgt"], g["src"]
+    e, t, s, empty = oracle.scale_masks(hole, 9)
+    assert bitexact(e, g["ext"]) and bitexact(t, tgt) and bitexact(s, src) and not empty
+    for kind in ["l2", "l1", "l2combined"]:
+        dp = oracle.dist_params(kind, 9, 10000.0, 0.3, 0.7)
+        assert bitexact(oracle.patch_distance_field(dp, img, img, nnf, tgt), g[f"dist_{kind}"]), kind
+    dp = oracle.dist_params("l2rgbd", 9, 10000.0, 0.3, 0.7)
+    assert bitexact(oracle.patch_distance_field(dp, img4, img4, nnf, tgt), g["dist_l2rgbd"])
+    assert bitexact(oracle.patchmatch(oracle.dist_params("l2", 9), img, src, img, tgt, None, seed=42),
+                    g["pm_l2_seed42"])
+    dpc = oracle.dist_params("l2combined", 9, 10000.0, 0.05)
+    assert bitexact(oracle.patchmatch(dpc, img, src, img, tgt, None, seed=42), g["pm_l2c_seed42"])
+    assert bitexact(oracle.patchmatch(dpc, img, src, img, tgt, g["pm_l2c_seed42"], seed=7), g["pm_l2c_init_seed7"])
+    pm, conf = g["pm_l2c_seed42"], g["conf"]
+    for kind in ["nlmeans", "nlmedians", "nlpoisson"]:
+        out, tot = oracle.update(kind, img, img, hole, g["ext"], pm, conf, lam=0.05, cg_tol=1e-4, cg_limit=100)
+        assert bitexact(out, g[f"upd_{kind}"]), kind
+        assert tot == float(g[f"upd_{kind}_total"]), kind
+    out, tot = oracle.update("nldepth", img4, img4, hole, g["ext"], g["pm4"], conf, lam=1.0, lam_d=0.0)
+    assert bitexact(out, g["upd_nldepth"]) and tot == float(g["upd_nldepth_total"])
+
+
+@pytest.mark.parametrize("method,init", [("nlmeans", "poisson"), ("nlmedians", "avg"), ("nlpoisson", "poisson"),
+                                         ("nlmeans", "none"), ("nlmeans", "black"), ("nldepth", "poisson")])
+def test_process_golden(oracle, method, init):
+    g = golden("process72x60")
+    im = g["img4"] if method == "nldepth" else np.ascontiguousarray(g["img4"][..., :3])
+    p = O.InpaintParams(method=method, patch=7, iters=6, scales=3, coarse=0.5, init=init, lam=0.05,
+                        lambda_rgb=1.0, lambda_d=0.0, cg_tol=1e-4, cg_limit=50, seed=11)
+    out, _ = oracle.process(im, g["hole"], p)
+    assert bitexact(out, g[f"process_{method}_{init}"])
+
+
+# ---------------------------------------------------------------- live reference
+def test_live_reference_pyramid_chain(oracle, reference, syn):
+    img = syn.texture(203, 151, 3, 5, 8)
+    hole = syn.ellipse_holes(203, 151, 5, 0.2, 8)
+    rate = oracle.subsampling_rate(0.1, 8)
+    for _ in range(7):
+        a = oracle.downsample_with_mask(img, hole, rate)
+        b = reference.downsample_with_mask(img, hole, rate)
+        assert bitexact(a[0], b[0]) and bitexact(a[1], b[1])
+        img, hole = a
+    for P in (5, 11):
+        assert bitexact(oracle.extended_domain(hole, P), reference.extended_domain(hole, P))
+
+
+def test_live_reference_patchmatch_and_updates(oracle, reference, syn):
+    W, H = 70, 58
+    img = syn.texture(W, H, 3, 3, 21)
+    hole = syn.ellipse_holes(W, H, 3, 0.18, 21, border=6)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 7)
+    dp = oracle.dist_params("l1", 7, 2.0)
+    a = oracle.patchmatch(dp, img, src, img, tgt, None, seed=5, iters=4, shots=12)
+    b = reference.patchmatch("l1", img, src, img, tgt, None, seed=5, patch=7, sigma=2.0, iters=4, shots=12)
+    assert bitexact(a, b)
+    conf = oracle.confidence(hole, 3.0, 0.2)
+    for kind in ["nlmeans", "nlmedians", "nlpoisson"]:
+        x, tx = oracle.update(kind, img, img, hole, ext, a, conf, patch=7, sigma=2.0, lam=0.3, cg_tol=1e-7,
+                              cg_limit=300)
+        y, ty = reference.update(kind, img, img, hole, ext, a, conf, patch=7, sigma=2.0, lam=0.3, cg_tol=1e-7,
+                                 cg_limit=300)
+        assert bitexact(x, y) and tx == ty, kind
+
+
+# ---------------------------------------------------------------- mathematics
+def test_gaussian_weights_closed_form(oracle):
+    for P in (5, 7, 9, 11):
+        w = oracle.gaussian_weights(P, P, 10000.0, 10000.0).astype(np.float64)
+        assert abs(w.sum() - 1.0) < 1e-6
+        assert np.allclose(w, w[::-1, ::-1]) and np.allclose(w, w.T)
+        assert np.max(np.abs(w - 1.0 / P ** 2)) < 1e-6           # sigma -> infinity
+    w = oracle.gaussian_weights(9, 1, 1.3, 0.0)[0].astype(np.float64)
+    x = np.arange(9) - 4.0
+    ref = np.exp(-(x ** 2) / np.float32(1.3) ** 2)                # 1-D form: no factor 2
+    assert np.allclose(w, ref / ref.sum(), rtol=1e-6)
+    w2 = oracle.gaussian_weights(7, 7, 1.5, 1.5).astype(np.float64)
+    x = np.arange(7) - 3.0
+    # 2-D: cell (i, j) evaluated at (x_i, y_j), exp(-(x^2 + y^2) / (2 sigma^2))
+    ref2 = np.exp(-(x[None, :] ** 2 + x[:, None] ** 2) / (2 * 1.5 ** 2))
+    assert np.allclose(w2, ref2 / ref2.sum(), rtol=1e-6)
+
+
+def test_gradient_forward_differences(oracle, syn):
+    img = syn.texture(31, 23, 3, 3, 2)
+    g = oracle.gradient(img)
+    gx = np.zeros_like(img)
+    gy = np.zeros_like(img)
+    gx[:, :-1] = img[:, 1:] - img[:, :-1]
+    gy[:-1, :] = img[1:, :] - img[:-1, :]
+    assert bitexact(g[..., 0::2], gx) and bitexact(g[..., 1::2], gy)
+
+
+def brute_edt(mask):
+    H, W = mask.shape
+    outside = np.argwhere(mask == 0)
+    out = np.zeros((H, W), np.float32)
+    for y, x in np.argwhere(mask):
+        d2 = (outside[:, 0] - y) ** 2 + (outside[:, 1] - x) ** 2
+        out[y, x] = np.float32(np.sqrt(np.float64(d2.min())))
+    return out
+
+
+def test_distance_transform_brute_force(oracle, syn):
+    for seed in range(3):
+        m = syn.ellipse_holes(41, 33, 3, 0.3, seed, border=1)
+        assert bitexact(oracle.distance_transform(m), brute_edt(m))
+
+
+def brute_extended(hole, P):
+    H, W = hole.shape
+    r = P // 2
+    out = hole.copy()
+    for y, x in np.argwhere(hole):
+        nb = [(x + 1, y), (x - 1, y), (x, y + 1), (x, y - 1)]
+        surrounded = all(0 <= a < W and 0 <= b < H and hole[b, a] for a, b in nb)
+        if not surrounded:
+            out[max(y - r, 0):y + r + 1, max(x - r, 0):x + r + 1] = 1
+    return out
+
+
+def test_masks_from_definitions(oracle, syn):
+    hole = syn.ellipse_holes(53, 47, 3, 0.2, 4, border=2)
+    for P in (3, 7, 9):
+        ext = oracle.extended_domain(hole, P)
+        assert bitexact(ext, brute_extended(hole, P))
+        e, t, s, empty = oracle.scale_masks(hole, P, erode=True)
+        r = P // 2
+        margin = np.zeros_like(hole, dtype=bool)
+        margin[:r, :] = margin[-r:, :] = True if r else False
+        if r:
+            margin[:, :r] = True
+            margin[:, -r:] = True
+        src0 = ext == 0
+        right = np.zeros_like(src0)
+        right[:, :-1] = src0[:, 1:]
+        down = np.zeros_like(src0)
+        down[:-1, :] = src0[1:, :]
+        eroded = src0 & right & down
+        assert bitexact(t, ((ext == 1) & ~margin).astype(np.uint8))
+        assert bitexact(s, (eroded & ~margin).astype(np.uint8))
+        assert empty == (not eroded.any())
+
+
+def test_patch_distance_fp64_brute_force(oracle, syn):
+    W, H = 40, 32
+    img = syn.texture(W, H, 3, 3, 9)
+    img4 = np.concatenate([img, syn.depth_planes(W, H, 5, 9)[..., None]], axis=2)
+    rng = np.random.default_rng(1)
+    P, r = 7, 3
+    tm = np.zeros((H, W), np.uint8)
+    tm[r:H - r, r:W - r] = 1
+    nnf = np.full((H, W, 2), -1, np.int32)
+    nnf[..., 0] = rng.integers(r, W - r, (H, W))
+    nnf[..., 1] = rng.integers(r, H - r, (H, W))
+    w = oracle.gaussian_weights(P, P, 2.5, 2.5).astype(np.float64)
+    for kind, im, lam, lamd in [("l2", img, 0, 0), ("l1", img, 0, 0), ("l2combined", img, 0.3, 0),
+                                ("l2rgbd", img4, 0.4, 0.2)]:
+        dp = oracle.dist_params(kind, P, 2.5, lam, lamd)
+        got = oracle.patch_distance_field(dp, im, im, nnf, tm)
+        g = oracle.gradient(im).astype(np.float64)
+        I = im.astype(np.float64)
+        for (y, x) in [(r, r), (H // 2, W // 2), (H - r - 1, W - r - 1), (10, 20)]:
+            sx, sy = nnf[y, x]
+            a = I[sy - r:sy + r + 1, sx - r:sx + r + 1]
+            b = I[y - r:y + r + 1, x - r:x + r + 1]
+            ga = g[sy - r:sy + r + 1, sx - r:sx + r + 1]
+            gb = g[y - r:y + r + 1, x - r:x + r + 1]
+            if kind == "l2":
+                ref = np.sum(w * np.sum((a - b) ** 2, axis=2))
+            elif kind == "l1":
+                ref = np.sum(w * np.sum(np.abs(a - b), axis=2))
+            elif kind == "l2combined":
+                ref = lam * np.sum(w * np.sum((a - b) ** 2, axis=2)) + \
+                    (1 - lam) * np.sum(w * np.sum((ga - gb) ** 2, axis=2))
+            else:
+                ref = lam * np.sum(w * np.sum((a[..., :3] - b[..., :3]) ** 2, axis=2)) + \
+                    (1 - lam) * np.sum(w * np.sum((ga[..., :6] - gb[..., :6]) ** 2, axis=2)) + \
+                    lamd * np.sum(w * (a[..., 3] - b[..., 3]) ** 2) + \
+                    (1 - lamd) * np.sum(w * np.sum((ga[..., 6:] - gb[..., 6:]) ** 2, axis=2))
+            ref /= P * P
+            assert abs(got[y, x] - ref) <= 2e-6 * ref + 1e-12, (kind, y, x, got[y, x], ref)
+
+
+def brute_nlmeans(img, hole, nnf, conf, w, P):
+    H, W, C = img.shape
+    h = P // 2
+    out = img.astype(np.float64).copy()
+    for y, x in np.argwhere(hole):
+        xa, xb, ya, yb = max(x - h, 0), min(x + h, W - 1), max(y - h, 0), min(y + h, H - 1)
+        tv = np.zeros(C)
+        tw = 0.0
+        for i in range(xa, xb + 1):
+            for j in range(ya, yb + 1):
+                cx, cy = nnf[j, i]
+                if cx == -1:
+                    continue
+                cx, cy = cx + x - i, cy + y - j
+                if not (0 <= cx < W and 0 <= cy < H):
+                    continue
+                wt = w[j - ya, i - xa] * (conf[j, i] if hole[j, i] else 1.0)   # clip-shifted weight index
+                tw += wt
+                tv += wt * img[cy, cx]
+        if tw > 0:
+            out[y, x] = tv / tw
+    return out
+
+
+def test_nlmeans_fp64_brute_force_and_invariants(oracle, syn):
+    W, H = 48, 40
+    img = syn.texture(W, H, 3, 3, 12)
+    hole = syn.ellipse_holes(W, H, 2, 0.15, 12, border=5)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 7)
+    nnf = oracle.patchmatch(oracle.dist_params("l2", 7, 3.0), img, src, img, tgt, None, seed=3, iters=3)
+    conf = oracle.confidence(hole, 5.0, 0.1)
+    w = oracle.gaussian_weights(7, 7, 3.0, 3.0).astype(np.float64)
+    out, _ = oracle.update("nlmeans", img, img, hole, ext, nnf, conf, patch=7, sigma=3.0)
+    ref = brute_nlmeans(img, hole, nnf, conf, w, 7)
+    assert np.max(np.abs(out - ref)) < 2e-6
+    const = np.full_like(img, 0.37)
+    out, tot = oracle.update("nlmeans", const, const, hole, ext, nnf, conf, patch=7, sigma=3.0)
+    assert np.max(np.abs(out - 0.37)) < 1e-6 and tot < 1e-10
+
+
+def test_weighted_median_definition(oracle, syn):
+    W, H = 40, 36
+    img = syn.texture(W, H, 3, 3, 14)
+    hole = syn.ellipse_holes(W, H, 2, 0.15, 14, border=5)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 5)
+    nnf = oracle.patchmatch(oracle.dist_params("l1", 5, 1.5), img, src, img, tgt, None, seed=3, iters=2)
+    conf = oracle.confidence(hole, 5.0, 0.1)
+    w = oracle.gaussian_weights(5, 5, 1.5, 1.5).astype(np.float64)
+    out, _ = oracle.update("nlmedians", img, img, hole, ext, nnf, conf, patch=5, sigma=1.5)
+    for y, x in np.argwhere(hole)[::7]:
+        vals, wts = [], []
+        xa, xb, ya, yb = max(x - 2, 0), min(x + 2, W - 1), max(y - 2, 0), min(y + 2, H - 1)
+        for i in range(xa, xb + 1):
+            for j in range(ya, yb + 1):
+                cx, cy = nnf[j, i]
+                if cx == -1:
+                    continue
+                cx, cy = cx + x - i, cy + y - j
+                if 0 <= cx < W and 0 <= cy < H:
+                    vals.append(img[cy, cx])
+                    wts.append(w[j - ya, i - xa] * (conf[j, i] if hole[j, i] else 1.0))
+        if not vals:
+            continue
+        vals, wts = np.array(vals), np.array(wts)
+        for c in range(3):
+            m = out[y, x, c]
+            assert m in vals[:, c]                       # an actual sample
+            below = wts[vals[:, c] < m].sum()
+            upto = wts[vals[:, c] <= m].sum()
+            # m is a weighted median: weight strictly below <= half <= weight up to m
+            assert below <= wts.sum() / 2 + 1e-6 and upto >= wts.sum() / 2 - 1e-6
+
+
+def poisson_matrix(a1, a2, hole):
+    """A = a1 - L over the hole pixels, L the anisotropic 5-point stencil
+    (bottom/right coefficient a2(centre), top/left a2(top/left))."""
+    H, W = hole.shape
+    pts = np.argwhere(hole)
+    idx = -np.ones((H, W), int)
+    idx[pts[:, 0], pts[:, 1]] = np.arange(len(pts))
+    rows, cols, vals = [], [], []
+    for k, (y, x) in enumerate(pts):
+        coef = {}
+        if y > 0:
+            coef[(y - 1, x)] = a2[y - 1, x]
+        if x > 0:
+            coef[(y, x - 1)] = a2[y, x - 1]
+        if y < H - 1:
+            coef[(y + 1, x)] = a2[y, x]
+        if x < W - 1:
+            coef[(y, x + 1)] = a2[y, x]
+        diag = a1[y, x] + sum(coef.values())
+        rows.append(k); cols.append(k); vals.append(diag)
+        for (yy, xx), c in coef.items():
+            if idx[yy, xx] >= 0:
+                rows.append(k); cols.append(idx[yy, xx]); vals.append(-c)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(len(pts), len(pts))), pts
+
+
+def brute_coefficients(img, ext, nnf, conf, w, P, lam):
+    H, W, C = img.shape
+    h = P // 2
+    lam1 = float(np.float32(1) - np.float32(lam))     # lambda is a float parameter
+    lam = float(np.float32(lam))
+    a1 = np.zeros((H, W)); a2 = np.zeros((H, W))
+    f1 = np.zeros((C, H, W)); F1 = np.zeros((C, H, W)); F2 = np.zeros((C, H, W))
+    g = np.zeros((H, W, 2 * C))
+    f32 = img.astype(np.float32)                       # forward differences are float32
+    g[:, :-1, 0::2] = f32[:, 1:] - f32[:, :-1]
+    g[:-1, :, 1::2] = f32[1:, :] - f32[:-1, :]
+    for py, px in np.argwhere(ext):
+        nx, ny = nnf[py, px]
+        for x in range(max(px - h, 0), min(px + h, W - 1) + 1):
+            for y in range(max(py - h, 0), min(py + h, H - 1) + 1):
+                n_x, n_y = nx + x - px, ny + y - py
+                if 0 <= n_x < W and 0 <= n_y < H:
+                    # centred weight index; the weight is a float32 product, as in the reference
+                    wt = float(np.float32(conf[py, px]) * np.float32(w[y - py + h, x - px + h]))
+                    a1[y, x] += wt * lam
+                    a2[y, x] += wt * lam1
+                    f1[:, y, x] += wt * img[n_y, n_x] * lam
+                    F1[:, y, x] += wt * g[n_y, n_x, 0::2] * lam1
+                    F2[:, y, x] += wt * g[n_y, n_x, 1::2] * lam1
+    f2 = np.zeros_like(f1)
+    for y, x in np.argwhere(ext):
+        dx = F1[:, y, x] if x == 0 else (-F1[:, y, x - 1] if x == W - 1 else F1[:, y, x] - F1[:, y, x - 1])
+        dy = F2[:, y, x] if y == 0 else (-F2[:, y - 1, x] if y == H - 1 else F2[:, y, x] - F2[:, y - 1, x])
+        f2[:, y, x] = dx + dy
+    return a1, a2, f1, f2
+
+
+def test_poisson_coefficients_brute_force(oracle, syn):
+    W, H = 30, 26
+    img = syn.texture(W, H, 3, 3, 15)
+    hole = syn.ellipse_holes(W, H, 2, 0.15, 15, border=4)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 5)
+    nnf = oracle.patchmatch(oracle.dist_params("l2combined", 5, 2.0, 0.2), img, src, img, tgt, None, seed=2, iters=2)
+    conf = oracle.confidence(hole, 5.0, 0.1)
+    w = oracle.gaussian_weights(5, 5, 2.0, 2.0).astype(np.float64)
+    a1, a2, f1, f2 = oracle.poisson_coefficients(img, ext, nnf, conf, patch=5, sigma=2.0, lam=0.2)
+    b1, b2, g1, g2 = brute_coefficients(img.astype(np.float64), ext, nnf, conf, w, 5, 0.2)
+    for x, y in [(a1, b1), (a2, b2), (f1, g1), (f2, g2)]:
+        assert np.max(np.abs(x - y)) <= 1e-12 * max(1.0, np.max(np.abs(y)))
+
+
+def test_cg_matches_direct_solve(oracle, syn):
+    W, H = 34, 28
+    img = syn.texture(W, H, 3, 3, 16)
+    hole = syn.ellipse_holes(W, H, 2, 0.2, 16, border=4)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 5)
+    nnf = oracle.patchmatch(oracle.dist_params("l2combined", 5, 2.0, 0.1), img, src, img, tgt, None, seed=2, iters=3)
+    conf = oracle.confidence(hole, 5.0, 0.1)
+    a1, a2, f1, f2 = oracle.poisson_coefficients(img, ext, nnf, conf, patch=5, sigma=2.0, lam=0.1)
+    A, pts = poisson_matrix(a1, a2, hole)
+    b = np.random.default_rng(0).normal(size=(H, W))
+    bb = b[pts[:, 0], pts[:, 1]]
+    x, it, nr0, nr = oracle.conjugate_gradient(np.zeros((H, W)), a1, a2, b, hole, 1e-20, 2000)
+    direct = spla.spsolve(A.tocsc(), bb)
+    assert np.max(np.abs(x[pts[:, 0], pts[:, 1]] - direct)) < 1e-8 * max(1, np.max(np.abs(direct)))
+    # the stopping rule: squared norms, at most limit + 1 iterations
+    x, it, nr0, nr = oracle.conjugate_gradient(np.zeros((H, W)), a1, a2, b, hole, 1e-4, 5)
+    assert it == 6
+    x, it, nr0, nr = oracle.conjugate_gradient(np.zeros((H, W)), a1, a2, b, hole, 1e-4, 1000)
+    assert nr < 1e-4 * nr0 and it < 1000
+
+
+def test_poisson_lambda_one_is_a_weighted_mean(oracle, syn):
+    # lambda = 1: a2 = 0 and f2 = 0, so A = diag(a1) and the solution is f1 / a1, the
+    # centred-weight average of the NNF contributions.
+    W, H = 36, 30
+    img = syn.texture(W, H, 3, 3, 17)
+    hole = syn.ellipse_holes(W, H, 2, 0.15, 17, border=4)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 5)
+    nnf = oracle.patchmatch(oracle.dist_params("l2", 5, 2.0), img, src, img, tgt, None, seed=4, iters=2)
+    conf = oracle.confidence(hole, 5.0, 0.1)
+    a1, a2, f1, f2 = oracle.poisson_coefficients(img, ext, nnf, conf, patch=5, sigma=2.0, lam=1.0)
+    assert np.all(a2 == 0) and np.all(f2 == 0)
+    out, _ = oracle.update("nlpoisson", img, img, hole, ext, nnf, conf, patch=5, sigma=2.0, lam=1.0, cg_tol=1e-12,
+                           cg_limit=50)
+    sel = hole.astype(bool)
+    expect = (f1 / np.where(a1 > 0, a1, 1))[:, sel].T
+    assert np.max(np.abs(out[sel] - expect)) < 1e-6
+
+
+def test_patchmatch_invariants(oracle, syn):
+    W, H = 60, 50
+    img = syn.texture(W, H, 3, 3, 18)
+    hole = syn.ellipse_holes(W, H, 3, 0.15, 18, border=6)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 7)
+    dp = oracle.dist_params("l2combined", 7, 10000.0, 0.05)
+    nnf, dist, energy = oracle.patchmatch(dp, img, src, img, tgt, None, seed=9, iters=6, with_metrics=True)
+    t = tgt.astype(bool)
+    assert np.all(nnf[~t] == -1)
+    ok = nnf[t][:, 0] >= 0
+    assert ok.all()
+    assert np.all(src[nnf[t][:, 1], nnf[t][:, 0]] == 1)             # every entry is a source point
+    assert bitexact(oracle.patch_distance_field(dp, img, img, nnf, tgt)[t], dist[t])
+    assert np.all(np.diff(energy) <= 1e-9)                          # energy never increases
+    # exhaustive nearest neighbour is a lower bound of the energy
+    w = oracle.gaussian_weights(7, 7, 10000.0, 10000.0)
+    sp_pts = np.argwhere(src)
+    sample = np.argwhere(t)[::37]
+    for y, x in sample:
+        best = np.inf
+        for sy, sx in sp_pts[::3]:
+            pass
+        f = np.full((H, W, 2), -1, np.int32)
+        m = np.zeros((H, W), np.uint8)
+        m[y, x] = 1
+        dmin = np.inf
+        for sy, sx in sp_pts:
+            f[y, x] = (sx, sy)
+            dmin = min(dmin, oracle.patch_distance_field(dp, img, img, f, m)[y, x])
+        assert dist[y, x] >= dmin
+
+
+def test_patchmatch_finds_exact_copies(oracle):
+    # a periodic texture has an exact copy of every patch: the search must reach (near) zero
+    H, W = 48, 64
+    y, x = np.mgrid[0:H, 0:W]
+    img = np.stack([(np.sin(2 * np.pi * x / 8) + 1) / 2, (np.cos(2 * np.pi * y / 8) + 1) / 2,
+                    ((x + y) % 8) / 8.0], axis=2).astype(np.float32)
+    hole = np.zeros((H, W), np.uint8)
+    hole[18:30, 24:40] = 1
+    ext, tgt, src, _ = oracle.scale_masks(hole, 5)
+    dp = oracle.dist_params("l2", 5, 10000.0)
+    nnf, dist, energy = oracle.patchmatch(dp, img, src, img, tgt, None, seed=1, iters=10, with_metrics=True)
+    assert np.median(dist[tgt.astype(bool)]) < 1e-10
+
+
+def test_simpois_approaches_harmonic_extension(oracle):
+    # the coarsest-level fill approximates the discrete Laplace (harmonic) extension
+    H, W = 40, 48
+    y, x = np.mgrid[0:H, 0:W].astype(np.float64)
+    img = (0.3 + 0.4 * x / W + 0.2 * y / H)[..., None].astype(np.float32)   # harmonic: exact answer is itself
+    hole = np.zeros((H, W), np.uint8)
+    hole[12:28, 14:34] = 1
+    out = oracle.simpois(img, hole)
+    assert np.max(np.abs(out - img)) < 0.02
+    assert np.all(out[hole == 0] == img[hole == 0])
+
+
+def test_subsampling_rate_and_sizes(oracle):
+    rate = oracle.subsampling_rate(0.1, 11)
+    assert abs(rate - 0.1 ** 0.1) < 1e-6
+    sizes = [1024]
+    for _ in range(10):
+        sizes.append(oracle.sample_size(sizes[-1], rate))
+    assert sizes == [1024, 813, 646, 513, 407, 323, 257, 204, 162, 129, 102]
+
+
+def test_process_empty_source_raises(oracle):
+    img = np.full((24, 24, 3), 0.5, np.float32)
+    hole = np.ones((24, 24), np.uint8)
+    with pytest.raises(RuntimeError, match="Empty source"):
+        oracle.process(img, hole, O.InpaintParams(method="nlmeans", patch=7, iters=2, scales=1, coarse=1.0))
