@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""bench.py -- inpainted MPix/s of the 1024x1024 nlpoisson workload on one MI355X.
+
+One step = one complete multiscale inpainting (ImageInpainting::process: 11
+scales, every inpainting iteration until the stopping rule, each a PatchMatch
+NNF search plus the NL-Poisson update with its CG solve) of one 1024x1024 RGB
+float32 image with a 15 % union-of-ellipses hole; lambda 0.05, patch 9, <= 200
+iterations per scale, CG tolerance 1e-4 with <= 100 iterations.  Inputs are
+resident in device memory before the timed region (device-resident driver,
+nli_inpaint_dev).  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "inpainted MPix/s at 1024² nlpoisson (batched FP32+FP64-refine vs plain, same GPU)"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def psnr(a, b):
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return float("inf") if mse == 0 else 10 * np.log10(1.0 / mse)
+
+
+def workload_params(nli, precision):
+    return nli.Inpainting(method="nlpoisson", patch=9, sigma=10000.0, lam=0.05, iters=200, tol=0.1, scales=11,
+                          coarse=0.1, conf_t=5.0, conf_a=0.1, init="poisson", pm_iters=10, pm_shots=30,
+                          pm_window=-1, cg_tol=1e-4, cg_limit=100, cg_precision=precision,
+                          cg_refine_every=8 if precision == nli.CG_MIXED else 0, seed=1)
+
+
+def cpu_baseline(wl, gpu_iterations, threads):
+    """The CPU oracle (the reference's algorithm, OpenMP PatchMatch) on a bounded sample
+    of the same workload: one inpainting iteration per scale, timed per scale; the
+    estimated full-run time multiplies each scale's per-iteration time by the number of
+    iterations the GPU run made at that scale."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    orc = O.Oracle()
+    p = O.InpaintParams(method="nlpoisson", patch=9, sigma=10000.0, lam=0.05, iters=200, tol=0.1, scales=11,
+                        coarse=0.1, init="poisson", cg_tol=1e-4, cg_limit=100, seed=1, pm_threads=threads,
+                        max_iters_per_scale=1)
+    t0 = time.perf_counter()
+    _, st = orc.process(wl.image, wl.mask, p)
+    sample_s = time.perf_counter() - t0
+    est = st.seconds_pyramid
+    for L in range(11):
+        est += st.seconds_propagate[L]
+        if st.iterations[L] > 0:
+            est += st.seconds_iterations[L] / st.iterations[L] * gpu_iterations[L]
+    H, W = wl.mask.shape
+    return dict(value=W * H / 1e6 / est, unit="MPix/s", cores=threads, kind="port",
+                sample=f"oracle process() of the same 1024x1024 workload with 1 inpainting iteration per scale "
+                       f"({sample_s:.1f} s, PatchMatch on {threads} OpenMP threads); estimated full run "
+                       f"{est:.0f} s = per-scale iteration time x the GPU run's iterations per scale")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (one process per GPU would "
+              f"inpaint independent images, see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
+
+    nli = importlib.import_module("non-local-inpainting_amd")
+    syn = importlib.import_module("non-local-inpainting_amd.synthetic")
+    wl = syn.config_1024()
+    H, W, C = wl.image.shape
+    precision = nli.CG_MIXED if args.precision == "mixed" else nli.CG_FP64
+    params = workload_params(nli, precision)
+    ctx = nli.Context(0)
+    d_in = ctx.to_device(wl.image)
+    d_mask = ctx.to_device(wl.mask)
+    d_out = ctx.dev_alloc(wl.image.nbytes)
+
+    for i in range(args.warmup):
+        st = ctx.inpaint_dev(d_in, d_mask, d_out, W, H, C, params)
+        log(f"warmup {i}: {st.seconds_total:.2f} s, iterations per scale {st.iterations}")
+
+    ctx.profile(True)
+    ctx.lib.nli_synchronize(ctx.h)
+    t0 = time.perf_counter()
+    runs = []
+    for i in range(args.steps):
+        st = ctx.inpaint_dev(d_in, d_mask, d_out, W, H, C, params)
+        runs.append(st)
+        log(f"step {i}: {st.seconds_total:.2f} s (PatchMatch {st.seconds_patchmatch:.2f} s, update "
+            f"{st.seconds_update:.2f} s), iterations per scale {st.iterations}, CG iterations "
+            f"{st.cg_iterations_total}")
+    ctx.lib.nli_synchronize(ctx.h)
+    elapsed = time.perf_counter() - t0
+    kern = {k: ctx.profile_read(k) for k in nli.KERNELS}
+    ctx.profile(False)
+    out = ctx.from_device(d_out, wl.image.shape, np.float32)
+    hole = wl.mask.astype(bool)
+    ms_per_step = elapsed / args.steps * 1e3
+    value = W * H / 1e6 / (ms_per_step / 1e3)
+    psnr_hole = psnr(out[hole], wl.truth[hole])
+
+    # roofline of the dominant kernel
+    sweep, cg = kern["pm_sweep"], kern["cg_iteration"]
+    if sweep["ms"] >= cg["ms"]:
+        achieved = sweep["work"] / sweep["ms"] / 1e9          # flops per ms -> TFLOP/s
+        roof = dict(kernel="k_pm_sweep (one PatchMatch sweep)", bound="mfma", achieved=achieved,
+                    peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS, traffic=None,
+                    launches=sweep["launches"], avg_launch_ms=sweep["ms"] / max(sweep["launches"], 1),
+                    note="FP32 vector work; no MFMA is used (patch distances are not a contraction)")
+    else:
+        achieved = cg["work"] / (cg["ms"] / 1e3) / 1e9
+        roof = dict(kernel="k_cg_a + k_cg_b (one CG iteration)", bound="hbm", achieved=achieved,
+                    peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS, traffic=None,
+                    launches=cg["launches"], avg_launch_ms=cg["ms"] / max(cg["launches"], 1))
+
+    compare = None
+    if not args.no_compare and precision == nli.CG_MIXED:
+        t1 = time.perf_counter()
+        st64 = ctx.inpaint_dev(d_in, d_mask, d_out, W, H, C, workload_params(nli, nli.CG_FP64))
+        out64 = ctx.from_device(d_out, wl.image.shape, np.float32)
+        compare = dict(fp64_mpix_s=W * H / 1e6 / (time.perf_counter() - t1),
+                       fp64_psnr_hole_db=psnr(out64[hole], wl.truth[hole]),
+                       fp64_iterations=st64.iterations,
+                       psnr_difference_db=psnr_hole - psnr(out64[hole], wl.truth[hole]))
+        log(f"FP64 comparison run: {compare}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"CPU baseline sample on {args.cpu_threads} threads ...")
+        cpu = cpu_baseline(wl, runs[-1].iterations, args.cpu_threads)
+        log(f"CPU baseline: {cpu}")
+
+    last = runs[-1]
+    line = {
+        "metric": METRIC, "value": value, "unit": "MPix/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": wl.name, "width": W, "height": H, "channels": C,
+                   "hole_fraction": float(wl.mask.mean()), "method": "nlpoisson", "lambda": 0.05, "patch": 9,
+                   "scales": 11, "max_iters": 200, "cg": "tol 1e-4, <= 100 iterations",
+                   "cg_precision": args.precision, "images": 1},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "psnr_hole_db": psnr_hole,
+        "iterations_per_scale": last.iterations,
+        "cg_iterations_total": last.cg_iterations_total,
+        "seconds": {"patchmatch": last.seconds_patchmatch, "update": last.seconds_update,
+                    "total": last.seconds_total},
+        "kernels": kern,
+        "fp64_compare": compare,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.dev_free(d_in)
+    ctx.dev_free(d_mask)
+    ctx.dev_free(d_out)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -227,6 +227,20 @@ int nli_inpaint_dev(nli_ctx *ctx, const nli_inpaint_params *params, const float 
                     const uint8_t *mask, int W, int H, int C, float *out,
                     nli_inpaint_stats *stats);
 
+/* ---- kernel timing ----
+ * With timing on, HIP events recorded on the context's stream bracket the launches of
+ * the kernels below; nli_profile_read synchronises and returns, for one kernel, the
+ * launches, their total duration, their algorithmic work (flops for PatchMatch sweeps,
+ * bytes for CG iterations) and the elements processed.  Enabling resets the totals. */
+#define NLI_KERNEL_PM_SWEEP 0
+#define NLI_KERNEL_PM_INIT 1
+#define NLI_KERNEL_CG_ITERATION 2
+#define NLI_KERNEL_POISSON_COEF 3
+#define NLI_KERNEL_NL_GATHER 4
+int nli_profile_enable(nli_ctx *ctx, int on);
+int nli_profile_read(nli_ctx *ctx, int kernel, int *launches, double *total_ms, double *work,
+                     double *units);
+
 /* fills params with the reference CLI defaults (main.cpp:68-82, 220-233, 254, 266) */
 void nli_inpaint_params_default(nli_inpaint_params *p);
 

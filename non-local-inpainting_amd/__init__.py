@@ -40,8 +40,10 @@ EXPORTS = [
     "nli_subsampling_rate", "nli_downsample_with_mask", "nli_downsample", "nli_downsample_mask",
     "nli_upsample", "nli_extended_domain", "nli_scale_masks", "nli_patch_distance",
     "nli_patchmatch", "nli_patchmatch_dev", "nli_update", "nli_update_dev", "nli_poisson_fill",
-    "nli_inpaint", "nli_inpaint_dev", "nli_inpaint_params_default",
+    "nli_inpaint", "nli_inpaint_dev", "nli_inpaint_params_default", "nli_profile_enable",
+    "nli_profile_read",
 ]
+KERNELS = {"pm_sweep": 0, "pm_init": 1, "cg_iteration": 2, "poisson_coef": 3, "nl_gather": 4}
 
 
 class NliError(RuntimeError):
@@ -345,6 +347,47 @@ class Context:
                                         _ptr(domain, _u8p), _ptr(ext, _u8p), _ptr(nnf, _i32p), _ptr(conf, _f32p),
                                         W, H, C, ct.byref(total), ct.byref(st)))
         return (img, total.value, st) if stats else (img, total.value)
+
+    # ---- kernel timing ----
+    def profile(self, on: bool = True):
+        self._check(self.lib.nli_profile_enable(self.h, int(on)))
+
+    def profile_read(self, kernel: str) -> dict:
+        n = ct.c_int()
+        ms, work, units = ct.c_double(), ct.c_double(), ct.c_double()
+        self._check(self.lib.nli_profile_read(self.h, KERNELS[kernel], ct.byref(n), ct.byref(ms), ct.byref(work),
+                                              ct.byref(units)))
+        return dict(launches=n.value, ms=ms.value, work=work.value, units=units.value)
+
+    # ---- device buffers ----
+    def dev_alloc(self, nbytes: int) -> int:
+        p = ct.c_void_p()
+        self._check(self.lib.nli_dev_alloc(self.h, ct.c_uint64(nbytes), ct.byref(p)))
+        return p.value
+
+    def dev_free(self, p: int):
+        self._check(self.lib.nli_dev_free(self.h, ct.c_void_p(p)))
+
+    def to_device(self, arr) -> int:
+        arr = np.ascontiguousarray(arr)
+        p = self.dev_alloc(arr.nbytes)
+        self._check(self.lib.nli_copy_h2d(self.h, ct.c_void_p(p), arr.ctypes.data_as(ct.c_void_p),
+                                          ct.c_uint64(arr.nbytes)))
+        return p
+
+    def from_device(self, p: int, shape, dtype):
+        out = np.empty(shape, dtype)
+        self._check(self.lib.nli_copy_d2h(self.h, out.ctypes.data_as(ct.c_void_p), ct.c_void_p(p),
+                                          ct.c_uint64(out.nbytes)))
+        return out
+
+    def inpaint_dev(self, d_in: int, d_mask: int, d_out: int, W: int, H: int, C: int, params: "Inpainting"):
+        """The device-resident process(): inputs and output already in device memory."""
+        p = params.to_struct()
+        st = InpaintStats()
+        self._check(self.lib.nli_inpaint_dev(self.h, ct.byref(p), ct.c_void_p(d_in), ct.c_void_p(d_mask), W, H, C,
+                                             ct.c_void_p(d_out), ct.byref(st)))
+        return stats_of(st)
 
     # ---- full process ----
     def inpaint(self, image, mask, params: Inpainting):

@@ -223,19 +223,17 @@ __global__ void k_gradient(const float *__restrict__ img, int W, int H, int C, f
     }
 }
 
-__global__ void k_tile_flags(const uint8_t *tmask, int W, int H, int ntx, int nty, uint8_t *flags)
+// flags[t] = tile t holds a target; *count += targets (the mask, without the border rule)
+__global__ void k_tile_flags(const uint8_t *tmask, int W, int H, int ntx, int nty, uint8_t *flags, int *count)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntx * nty) return;
     int x0 = (t % ntx) * TX, y0 = (t / ntx) * TY;
-    uint8_t any = 0;
-    for (int y = y0; y < min(y0 + TY, H) && !any; y++)
-        for (int x = x0; x < min(x0 + TX, W); x++)
-            if (tmask[(size_t)W * y + x]) {
-                any = 1;
-                break;
-            }
-    flags[t] = any;
+    int n = 0;
+    for (int y = y0; y < min(y0 + TY, H); y++)
+        for (int x = x0; x < min(x0 + TX, W); x++) n += tmask[(size_t)W * y + x] ? 1 : 0;
+    flags[t] = n > 0;
+    if (n) atomicAdd(count, n);
 }
 
 // Initialisation (patch_match.cpp:94-141): the given entry, or uniform draws until one
@@ -382,9 +380,6 @@ __global__ __launch_bounds__(kBlock) void k_dist_field(PMArgs a, const int2 *nnf
 }
 
 // ---- dispatch over (kind, channels) ----
-template <template <int, int> class K>
-struct Dispatch;
-
 #define NLI_DISPATCH(KIND, C, CALL)                                                           \
     switch (KIND) {                                                                           \
     case DIST_L2:                                                                             \
@@ -424,6 +419,7 @@ struct Dispatch;
 struct Prepared {
     PMArgs a;
     int ntiles = 0;
+    int targets = 0;   // read back only when kernel timing is enabled
 };
 
 Prepared prepare(const DistDesc &d, const float *src, const uint8_t *smask, int sW, int sH, const float *tgt,
@@ -467,12 +463,18 @@ Prepared prepare(const DistDesc &d, const float *src, const uint8_t *smask, int 
         int ntx = (tW + TX - 1) / TX, nty = (tH + TY - 1) / TY;
         int nt = ntx * nty;
         w.tflags.ensure(nt);
-        k_tile_flags<<<(nt + kBlock - 1) / kBlock, kBlock, 0, s>>>(tmask, tW, tH, ntx, nty, w.tflags.p);
+        w.total.ensure(2);
+        NLI_HIP(hipMemsetAsync(w.total.p + 1, 0, sizeof(int), s));
+        k_tile_flags<<<(nt + kBlock - 1) / kBlock, kBlock, 0, s>>>(tmask, tW, tH, ntx, nty, w.tflags.p,
+                                                                    w.total.p + 1);
         NLI_LAUNCH_CHECK();
         w.tiles.ensure(nt);
         w.counts.ensure(nt / 1024 + 2);
-        w.total.ensure(1);
         pr.ntiles = compact_indices(w.tflags.p, nt, w.tiles.p, w.counts.p, w.total.p, s);
+        if (prof::enabled()) {
+            NLI_HIP(hipMemcpyAsync(&pr.targets, w.total.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+            NLI_HIP(hipStreamSynchronize(s));
+        }
         a.tiles = w.tiles.p;
         a.ntiles_x = ntx;
     }
@@ -532,16 +534,24 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     a.nnf_out = n_cur;
     a.dist_out = d_cur;
     a.stream = 0xFFFFu;
+    prof::begin(TK_PM_INIT, s);
 #define NLI_INIT(KD, CC) k_pm_init<KD, CC><<<pr.ntiles, kBlock, 0, s>>>(a)
     NLI_DISPATCH(d.kind, C, NLI_INIT);
 #undef NLI_INIT
     NLI_LAUNCH_CHECK();
+    prof::end(TK_PM_INIT, s, 0.0, pr.targets);
     const int r = d.patch / 2;
     const int F = (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) ? 3 * C : C;
     const size_t lds = ((size_t)F * (TX + 2 * r) * (TY + 2 * r) + (size_t)d.patch * d.patch) * sizeof(float);
     const int s0 = cfg.first_step > 0 ? cfg.first_step : auto_first_step(tW, tH);
     int levels = 0;
     while ((s0 >> levels) > 1) levels++;
+    // algorithmic flops of one sweep: T targets x (2 + R) candidates x P^2 pixels x
+    // (9C + 4) (gradient distances) or (3C + 2) (colour-only distances)
+    int R = 0;
+    for (int ws = a.maxw; ws >= 1; ws /= 2) R++;
+    const double per_pixel = (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) ? 9.0 * C + 4 : 3.0 * C + 2;
+    const double sweep_flops = (double)pr.targets * (2 + R) * d.patch * d.patch * per_pixel;
     for (int k = 0; k < K; k++) {
         a.nnf_in = n_cur;
         a.dist_in = d_cur;
@@ -549,10 +559,12 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
         a.dist_out = d_oth;
         a.step = s0 >> (k % (levels + 1));
         a.stream = (uint32_t)k;
+        prof::begin(TK_PM_SWEEP, s);
 #define NLI_SWEEP(KD, CC) k_pm_sweep<KD, CC><<<pr.ntiles, kBlock, lds, s>>>(a)
         NLI_DISPATCH(d.kind, C, NLI_SWEEP);
 #undef NLI_SWEEP
         NLI_LAUNCH_CHECK();
+        prof::end(TK_PM_SWEEP, s, sweep_flops, pr.targets);
         std::swap(n_cur, n_oth);
         std::swap(d_cur, d_oth);
     }

@@ -756,10 +756,15 @@ void run_cg(const CGParams &cg, int n, UpdWork &w, double *x0, hipStream_t s, CG
         k_cg_init<NC><<<nb, kBlock, 0, s>>>(a);
         NLI_LAUNCH_CHECK();
         const int poll = 8;
+        // algorithmic bytes of one iteration of the reference recurrence (two matvecs):
+        // 2 x 13 words x 8 bytes per unknown and channel
+        const double it_bytes = 2.0 * 13 * 8 * NC * (double)n;
         for (int it = 0; it <= cg.limit; it++) {
+            prof::begin(TK_CG, s);
             k_cg_a<NC><<<nb, kBlock, 0, s>>>(a, it);
             k_cg_b<NC><<<nb, kBlock, 0, s>>>(a, it);
             NLI_LAUNCH_CHECK();
+            prof::end(TK_CG, s, it_bytes, (double)NC * n);
             if ((it + 1) % poll == 0 && it < cg.limit) {
                 double flags[kMaxNC * S_COUNT];
                 NLI_HIP(hipMemcpyAsync(flags, scal, sizeof(double) * NC * S_COUNT, hipMemcpyDeviceToHost, s));
@@ -791,10 +796,13 @@ void run_cg(const CGParams &cg, int n, UpdWork &w, double *x0, hipStream_t s, CG
         k_cgf_init<NC><<<nb, kBlock, 0, s>>>(a, x0);
         NLI_LAUNCH_CHECK();
         const int poll = 8;
+        const double it_bytes = 2.0 * 13 * 4 * NC * (double)n;   // FP32 vectors
         for (int it = 0; it <= cg.limit; it++) {
+            prof::begin(TK_CG, s);
             k_cgf_a<NC><<<nb, kBlock, 0, s>>>(a, it);
             k_cgf_b<NC><<<nb, kBlock, 0, s>>>(a, it);
             NLI_LAUNCH_CHECK();
+            prof::end(TK_CG, s, it_bytes, (double)NC * n);
             if ((it + 1) % poll == 0 && it < cg.limit) {
                 double flags[kMaxNC * S_COUNT];
                 NLI_HIP(hipMemcpyAsync(flags, scal, sizeof(double) * NC * S_COUNT, hipMemcpyDeviceToHost, s));
@@ -832,9 +840,11 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     w.F2.ensure(np * nc);
     w.f2.ensure(np * nc);
     dim3 g2((W + kBlock - 1) / kBlock, H);
+    prof::begin(TK_POISSON_COEF, s);
     k_poisson_coef<<<g2, kBlock, 0, s>>>(image, w.grad.p, Cs, c0, nc, ext, (const int2 *)nnf, conf,
                                          w.weights.p, patch, lambda, W, H, w.a1.p, w.a2.p, w.f1.p, w.F1.p, w.F2.p);
     NLI_LAUNCH_CHECK();
+    prof::end(TK_POISSON_COEF, s, 0.0, (double)np);
     k_divergence<<<g2, kBlock, 0, s>>>(ext, w.F1.p, w.F2.p, nc, W, H, lambda != 1 ? 1 : 0, w.f2.p);
     NLI_LAUNCH_CHECK();
     const int nb = (int)grid_for(std::max(n, 1));
@@ -912,6 +922,7 @@ double update(int kind, int patch, float sigma, float lambda, const CGParams &cg
         w.x1.ensure(((size_t)n * nc + 1) / 2 + 1);   // new values (floats) live in x1's storage
         float *newv = (float *)w.x1.p;
         w.pflag.ensure(n);
+        prof::begin(TK_NL_GATHER, s);
         if (kind == UPD_MEANS) {
             k_nlmeans<<<grid_for(n), kBlock, 0, s>>>(image, Cs, c0, nc, hole_pts, n, hole, (const int2 *)nnf, conf,
                                                      w.weights.p, patch, W, H, newv, w.pflag.p);
@@ -923,6 +934,7 @@ double update(int kind, int patch, float sigma, float lambda, const CGParams &cg
                                                            conf, w.weights.p, patch, W, H, newv, w.pflag.p);
         }
         NLI_LAUNCH_CHECK();
+        prof::end(TK_NL_GATHER, s, 0.0, (double)n);
         const int nb = (int)grid_for(n);
         w.scalars.ensure(kMaxNC * S_COUNT + 8);
         w.partials.ensure((size_t)kMaxNC * nb + 8);

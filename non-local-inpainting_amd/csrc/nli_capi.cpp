@@ -285,6 +285,23 @@ int nli_copy_d2h(nli_ctx *ctx, void *dst, const void *src, uint64_t bytes)
     });
 }
 
+int nli_profile_enable(nli_ctx *ctx, int on)
+{
+    return guard(ctx, [&] {
+        sync(ctx);
+        nli::prof::enable(on != 0);
+    });
+}
+
+int nli_profile_read(nli_ctx *ctx, int kernel, int *launches, double *total_ms, double *work, double *units)
+{
+    return guard(ctx, [&] {
+        need(kernel >= 0 && kernel < nli::TK_COUNT, "unknown kernel id");
+        sync(ctx);
+        nli::prof::read(kernel, launches, total_ms, work, units);
+    });
+}
+
 int nli_sample_size(int size, float factor) { return nli::sample_size(size, factor); }
 float nli_subsampling_rate(float ratio, int scales) { return nli::subsampling_rate(ratio, scales); }
 

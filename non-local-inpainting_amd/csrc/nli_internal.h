@@ -82,6 +82,27 @@ struct DistDesc {
     float lambda_d;
 };
 
+// ---- kernel timing (HIP events on the launching stream), off unless enabled ----
+enum TimedKernel {
+    TK_PM_SWEEP = 0,    // k_pm_sweep: one PatchMatch sweep
+    TK_PM_INIT = 1,     // k_pm_init
+    TK_CG = 2,          // one CG iteration (phase A + phase B kernels)
+    TK_POISSON_COEF = 3,
+    TK_NL_GATHER = 4,   // NL-means / NL-medians gathers
+    TK_PYRAMID = 5,
+    TK_COUNT = 6
+};
+namespace prof {
+bool enabled();
+void enable(bool on);
+// brackets launches on stream s; `work` is the algorithmic flops or bytes of the
+// bracketed launches, `units` the elements they process
+void begin(int kernel, hipStream_t s);
+void end(int kernel, hipStream_t s, double work, double units);
+// synchronises and returns launches, total milliseconds, total work, total units
+void read(int kernel, int *launches, double *ms, double *work, double *units);
+}  // namespace prof
+
 // ---- patch weights / filters (host side, double libm as the reference) ----
 void gaussian_weights(int size_x, int size_y, float sigma_x, float sigma_y, float *out);
 float subsampling_rate(float ratio, int scales);

@@ -1,0 +1,119 @@
+// nli_prof.cpp -- optional per-kernel timing with HIP events recorded on the
+// stream the kernels are launched on (the event pair brackets the launches).
+#include "nli_internal.h"
+
+#include <mutex>
+#include <vector>
+
+namespace nli {
+namespace prof {
+namespace {
+
+struct Rec {
+    int kernel;
+    hipEvent_t a, b;
+    double work, units;
+};
+
+struct State {
+    std::mutex mu;
+    bool on = false;
+    std::vector<Rec> open;           // begun, not ended (per kernel at most one)
+    std::vector<Rec> done;
+    std::vector<hipEvent_t> pool;
+    int launches[TK_COUNT] = {};
+    double ms[TK_COUNT] = {}, work[TK_COUNT] = {}, units[TK_COUNT] = {};
+};
+
+State &st()
+{
+    static State s;
+    return s;
+}
+
+hipEvent_t take(State &s)
+{
+    if (!s.pool.empty()) {
+        hipEvent_t e = s.pool.back();
+        s.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    NLI_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void drain(State &s)
+{
+    for (Rec &r : s.done) {
+        NLI_HIP(hipEventSynchronize(r.b));
+        float ms = 0;
+        NLI_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        s.launches[r.kernel] += 1;
+        s.ms[r.kernel] += ms;
+        s.work[r.kernel] += r.work;
+        s.units[r.kernel] += r.units;
+        s.pool.push_back(r.a);
+        s.pool.push_back(r.b);
+    }
+    s.done.clear();
+}
+
+}  // namespace
+
+bool enabled() { return st().on; }
+
+void enable(bool on)
+{
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    drain(s);
+    s.on = on;
+    for (int k = 0; k < TK_COUNT; k++) {
+        s.launches[k] = 0;
+        s.ms[k] = s.work[k] = s.units[k] = 0;
+    }
+}
+
+void begin(int kernel, hipStream_t stream)
+{
+    State &s = st();
+    if (!s.on) return;
+    std::lock_guard<std::mutex> g(s.mu);
+    Rec r{kernel, take(s), take(s), 0, 0};
+    NLI_HIP(hipEventRecord(r.a, stream));
+    s.open.push_back(r);
+}
+
+void end(int kernel, hipStream_t stream, double work, double units)
+{
+    State &s = st();
+    if (!s.on) return;
+    std::lock_guard<std::mutex> g(s.mu);
+    for (size_t i = 0; i < s.open.size(); i++)
+        if (s.open[i].kernel == kernel) {
+            Rec r = s.open[i];
+            s.open.erase(s.open.begin() + i);
+            NLI_HIP(hipEventRecord(r.b, stream));
+            r.work = work;
+            r.units = units;
+            s.done.push_back(r);
+            if (s.done.size() > 4096) drain(s);
+            return;
+        }
+}
+
+void read(int kernel, int *launches, double *ms, double *work, double *units)
+{
+    State &s = st();
+    std::lock_guard<std::mutex> g(s.mu);
+    drain(s);
+    if (kernel < 0 || kernel >= TK_COUNT) throw std::invalid_argument("unknown timed kernel");
+    if (launches) *launches = s.launches[kernel];
+    if (ms) *ms = s.ms[kernel];
+    if (work) *work = s.work[kernel];
+    if (units) *units = s.units[kernel];
+}
+
+}  // namespace prof
+}  // namespace nli

@@ -1526,6 +1526,7 @@ int nlo_process(const nlo_process_params *p, const float *in, const uint8_t *mas
         fvec original = pyr[level];
         double total = std::numeric_limits<double>::max();
         int i = 0;
+        const double t_level = now_s();
         ivec next(2 * n);
         for (i = 0; i < cap && total > p->tol; i++) {
             run_pm(pyr[level].data(), srcm.data(), tgt.data(), w, h, nnf.empty() ? nullptr : nnf.data(),
@@ -1535,6 +1536,7 @@ int nlo_process(const nlo_process_params *p, const float *in, const uint8_t *mas
                             conf, w, h);
         }
         st->iterations[level] = i;
+        st->seconds_iterations[level] = now_s() - t_level;
         return 0;
     };
 
@@ -1546,6 +1548,7 @@ int nlo_process(const nlo_process_params *p, const float *in, const uint8_t *mas
         int uw = ws[i], uh = hs[i], lw = ws[i + 1], lh = hs[i + 1];
         size_t un = (size_t)uw * uh, ln = (size_t)lw * lh;
         conf.assign(un, 0.0f);
+        const double t_prop = now_s();
         nlo_confidence(mpyr[i].data(), uw, uh, p->conf_t, p->conf_a, conf.data());
         /* propagate_weights, :492-557 */
         bvec lext(ln), ltgt(ln), lsrc(ln);
@@ -1567,6 +1570,7 @@ int nlo_process(const nlo_process_params *p, const float *in, const uint8_t *mas
             run_upd(pyr[i].data(), orig_copy.data(), mpyr[i].data(), uext.data(), rnnf.data(), conf.data(), uw, uh);
         }
         nnf = rnnf;
+        st->seconds_propagate[i] = now_s() - t_prop;
         if (inpaint_internal(i, conf.data(), nnf) != 0) return -2;
     }
     std::memcpy(out, pyr[0].data(), (size_t)W * H * C * sizeof(float));

@@ -89,6 +89,8 @@ typedef struct {
     int iterations[32];      /* inpainting iterations run per scale, finest = index 0 */
     int width[32], height[32];
     double seconds_pyramid, seconds_patchmatch, seconds_update, seconds_other;
+    double seconds_iterations[32];   /* per scale: the inpainting iterations */
+    double seconds_propagate[32];    /* per scale: propagate_weights into it */
 } nlo_process_stats;
 
 /* ---- numerics ---- */

@@ -70,7 +70,8 @@ class ProcessParams(ct.Structure):
 class ProcessStats(ct.Structure):
     _fields_ = [("iterations", ct.c_int * 32), ("width", ct.c_int * 32), ("height", ct.c_int * 32),
                 ("seconds_pyramid", ct.c_double), ("seconds_patchmatch", ct.c_double),
-                ("seconds_update", ct.c_double), ("seconds_other", ct.c_double)]
+                ("seconds_update", ct.c_double), ("seconds_other", ct.c_double),
+                ("seconds_iterations", ct.c_double * 32), ("seconds_propagate", ct.c_double * 32)]
 
 
 @dataclass
