@@ -119,17 +119,18 @@ struct TgtGlobal {
 
 // Patch distance between the source patch centred at (sx, sy) and the target patch of
 // `t`.  Returns +inf as soon as the partial sum proves the result is >= bd.
-template <int KIND, int C, class T>
+template <int KIND, int C, int PS, class T>
 __device__ float patch_distance(const PMArgs &a, const float *__restrict__ w, int sx, int sy, const T &t, float bd)
 {
-    const int P = a.P, r = P / 2;
+    const int P = PS ? PS : a.P, r = P / 2;
     const unsigned PP = (unsigned)(P * P);
     if constexpr (KIND == DIST_L2 || KIND == DIST_L1) {
         const float bound = bd * (float)PP * (1.0f + 0x1p-20f);
         float acc = 0.0f;
         for (int dy = 0; dy < P; dy++) {
             const float *srow = a.src + ((size_t)(sy - r + dy) * a.sW + (sx - r)) * C;
-            for (int dx = 0; dx < P; dx++) {
+#pragma unroll
+            for (int dx = 0; dx < (PS ? PS : P); dx++) {
                 float norm = 0.0f;
 #pragma unroll
                 for (int c = 0; c < C; c++) {
@@ -151,7 +152,8 @@ __device__ float patch_distance(const PMArgs &a, const float *__restrict__ w, in
             const size_t rowp = (size_t)(sy - r + dy) * a.sW + (sx - r);
             const float *srow = a.src + rowp * C;
             const float *grow = a.sgrad + rowp * 2 * C;
-            for (int dx = 0; dx < P; dx++) {
+#pragma unroll
+            for (int dx = 0; dx < (PS ? PS : P); dx++) {
                 const float wt = w[dy * P + dx];
                 if constexpr (KIND == DIST_L2COMBINED && (C == 1 || C == 3)) {
                     float norm = 0.0f, gnorm = 0.0f;
@@ -238,7 +240,7 @@ __global__ void k_tile_flags(const uint8_t *tmask, int W, int H, int ntx, int nt
 
 // Initialisation (patch_match.cpp:94-141): the given entry, or uniform draws until one
 // lands in the source (at most `shots`), then its distance.
-template <int KIND, int C>
+template <int KIND, int C, int PS>
 __global__ __launch_bounds__(kBlock) void k_pm_init(PMArgs a)
 {
     __shared__ float wl[15 * 15];
@@ -273,23 +275,24 @@ __global__ __launch_bounds__(kBlock) void k_pm_init(PMArgs a)
     if (is_source(a, cx, cy)) {
         TgtGlobal<C> t{a.tgt, a.tgrad, a.tW, x - r, y - r};
         a.nnf_out[p] = make_int2(cx, cy);
-        a.dist_out[p] = patch_distance<KIND, C>(a, wl, cx, cy, t, FLT_MAX);
+        a.dist_out[p] = patch_distance<KIND, C, PS>(a, wl, cx, cy, t, FLT_MAX);
     }
 }
 
 // One sweep: jump-flood propagation at distance `step` + random search.
-template <int KIND, int C>
+template <int KIND, int C, int PS>
 __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
 {
     extern __shared__ float lds[];
     constexpr int F = Feat<KIND>::grad ? 3 * C : C;
-    const int r = a.P / 2;
+    const int P = PS ? PS : a.P;
+    const int r = P / 2;
     const int LW = TX + 2 * r, LH = TY + 2 * r;
     const int plane = LW * LH;
     float *wl = lds + F * plane;
     const int tile = a.tiles[blockIdx.x];
     const int x0 = (tile % a.ntiles_x) * TX, y0 = (tile / a.ntiles_x) * TY;
-    for (int i = threadIdx.x; i < a.P * a.P; i += blockDim.x) wl[i] = a.w[i];
+    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = a.w[i];
     for (int i = threadIdx.x; i < plane; i += blockDim.x) {
         int gx = x0 - r + i % LW, gy = y0 - r + i / LW;
         bool in = (unsigned)gx < (unsigned)a.tW && (unsigned)gy < (unsigned)a.tH;
@@ -321,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
         c.x -= ox[k];
         c.y -= oy[k];
         if (!is_source(a, c.x, c.y)) continue;
-        float d = patch_distance<KIND, C>(a, wl, c.x, c.y, t, bd);
+        float d = patch_distance<KIND, C, PS>(a, wl, c.x, c.y, t, bd);
         if (d < bd) {
             bd = d;
             best = c;
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
             }
         }
         if (found) {
-            float d = patch_distance<KIND, C>(a, wl, cx, cy, t, bd);
+            float d = patch_distance<KIND, C, PS>(a, wl, cx, cy, t, bd);
             if (d < bd) {
                 bd = d;
                 best = make_int2(cx, cy);
@@ -355,7 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
     a.dist_out[p] = bd;
 }
 
-template <int KIND, int C>
+template <int KIND, int C, int PS>
 __global__ __launch_bounds__(kBlock) void k_dist_field(PMArgs a, const int2 *nnf)
 {
     __shared__ float wl[15 * 15];
@@ -372,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_dist_field(PMArgs a, const int2 *nnf
             int r = a.P / 2;
             if (c.x >= r && c.y >= r && c.x < a.sW - r && c.y < a.sH - r) {
                 TgtGlobal<C> t{a.tgt, a.tgrad, a.tW, x - r, y - r};
-                out = patch_distance<KIND, C>(a, wl, c.x, c.y, t, FLT_MAX);
+                out = patch_distance<KIND, C, PS>(a, wl, c.x, c.y, t, FLT_MAX);
             }
         }
     }
@@ -380,40 +383,60 @@ __global__ __launch_bounds__(kBlock) void k_dist_field(PMArgs a, const int2 *nnf
 }
 
 // ---- dispatch over (kind, channels) ----
-#define NLI_DISPATCH(KIND, C, CALL)                                                           \
+// Kernel variants: the patch side is a compile-time constant for the common
+// (distance, channels, patch) triples and a runtime value otherwise.
+#define NLI_DISPATCH_RT(KIND, C, CALL)                                                        \
     switch (KIND) {                                                                           \
     case DIST_L2:                                                                             \
         switch (C) {                                                                          \
-        case 1: CALL(DIST_L2, 1); break;                                                      \
-        case 2: CALL(DIST_L2, 2); break;                                                      \
-        case 3: CALL(DIST_L2, 3); break;                                                      \
-        case 4: CALL(DIST_L2, 4); break;                                                      \
+        case 1: CALL(DIST_L2, 1, 0); break;                                                   \
+        case 2: CALL(DIST_L2, 2, 0); break;                                                   \
+        case 3: CALL(DIST_L2, 3, 0); break;                                                   \
+        case 4: CALL(DIST_L2, 4, 0); break;                                                   \
         default: throw std::invalid_argument("patch distance: 1 to 4 channels");              \
         }                                                                                     \
         break;                                                                                \
     case DIST_L1:                                                                             \
         switch (C) {                                                                          \
-        case 1: CALL(DIST_L1, 1); break;                                                      \
-        case 2: CALL(DIST_L1, 2); break;                                                      \
-        case 3: CALL(DIST_L1, 3); break;                                                      \
-        case 4: CALL(DIST_L1, 4); break;                                                      \
+        case 1: CALL(DIST_L1, 1, 0); break;                                                   \
+        case 2: CALL(DIST_L1, 2, 0); break;                                                   \
+        case 3: CALL(DIST_L1, 3, 0); break;                                                   \
+        case 4: CALL(DIST_L1, 4, 0); break;                                                   \
         default: throw std::invalid_argument("patch distance: 1 to 4 channels");              \
         }                                                                                     \
         break;                                                                                \
     case DIST_L2COMBINED:                                                                     \
         switch (C) {                                                                          \
-        case 1: CALL(DIST_L2COMBINED, 1); break;                                              \
-        case 2: CALL(DIST_L2COMBINED, 2); break;                                              \
-        case 3: CALL(DIST_L2COMBINED, 3); break;                                              \
-        case 4: CALL(DIST_L2COMBINED, 4); break;                                              \
+        case 1: CALL(DIST_L2COMBINED, 1, 0); break;                                           \
+        case 2: CALL(DIST_L2COMBINED, 2, 0); break;                                           \
+        case 3: CALL(DIST_L2COMBINED, 3, 0); break;                                           \
+        case 4: CALL(DIST_L2COMBINED, 4, 0); break;                                           \
         default: throw std::invalid_argument("patch distance: 1 to 4 channels");              \
         }                                                                                     \
         break;                                                                                \
     case DIST_L2RGBD:                                                                         \
         if (C != 4) throw std::invalid_argument("L2 RGBD Distance: Channels do not match.");  \
-        CALL(DIST_L2RGBD, 4);                                                                 \
+        CALL(DIST_L2RGBD, 4, 0);                                                              \
         break;                                                                                \
     default: throw std::invalid_argument("unknown patch distance kind");                      \
+    }
+
+#define NLI_SPECIAL(KIND, C, P, K, CC, PP, CALL)                                              \
+    if ((KIND) == (K) && (C) == (CC) && (P) == (PP)) {                                        \
+        CALL(K, CC, PP);                                                                      \
+    } else
+
+#define NLI_DISPATCH(KIND, C, P, CALL)                                                        \
+    NLI_SPECIAL(KIND, C, P, DIST_L2COMBINED, 3, 9, CALL)                                      \
+    NLI_SPECIAL(KIND, C, P, DIST_L2COMBINED, 3, 7, CALL)                                      \
+    NLI_SPECIAL(KIND, C, P, DIST_L2COMBINED, 3, 11, CALL)                                     \
+    NLI_SPECIAL(KIND, C, P, DIST_L2, 3, 9, CALL)                                              \
+    NLI_SPECIAL(KIND, C, P, DIST_L2, 3, 7, CALL)                                              \
+    NLI_SPECIAL(KIND, C, P, DIST_L1, 3, 9, CALL)                                              \
+    NLI_SPECIAL(KIND, C, P, DIST_L1, 3, 7, CALL)                                              \
+    NLI_SPECIAL(KIND, C, P, DIST_L2RGBD, 4, 9, CALL)                                          \
+    {                                                                                         \
+        NLI_DISPATCH_RT(KIND, C, CALL)                                                        \
     }
 
 struct Prepared {
@@ -535,8 +558,8 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     a.dist_out = d_cur;
     a.stream = 0xFFFFu;
     prof::begin(TK_PM_INIT, s);
-#define NLI_INIT(KD, CC) k_pm_init<KD, CC><<<pr.ntiles, kBlock, 0, s>>>(a)
-    NLI_DISPATCH(d.kind, C, NLI_INIT);
+#define NLI_INIT(KD, CC, PP) k_pm_init<KD, CC, PP><<<pr.ntiles, kBlock, 0, s>>>(a)
+    NLI_DISPATCH(d.kind, C, d.patch, NLI_INIT);
 #undef NLI_INIT
     NLI_LAUNCH_CHECK();
     prof::end(TK_PM_INIT, s, 0.0, pr.targets);
@@ -560,8 +583,8 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
         a.step = s0 >> (k % (levels + 1));
         a.stream = (uint32_t)k;
         prof::begin(TK_PM_SWEEP, s);
-#define NLI_SWEEP(KD, CC) k_pm_sweep<KD, CC><<<pr.ntiles, kBlock, lds, s>>>(a)
-        NLI_DISPATCH(d.kind, C, NLI_SWEEP);
+#define NLI_SWEEP(KD, CC, PP) k_pm_sweep<KD, CC, PP><<<pr.ntiles, kBlock, lds, s>>>(a)
+        NLI_DISPATCH(d.kind, C, d.patch, NLI_SWEEP);
 #undef NLI_SWEEP
         NLI_LAUNCH_CHECK();
         prof::end(TK_PM_SWEEP, s, sweep_flops, pr.targets);
@@ -578,8 +601,8 @@ void patch_distance_field(const DistDesc &d, const float *src, int sW, int sH, c
     PMArgs a = pr.a;
     a.dist_out = out;
     dim3 g((tW + kBlock - 1) / kBlock, tH);
-#define NLI_FIELD(KD, CC) k_dist_field<KD, CC><<<g, kBlock, 0, s>>>(a, (const int2 *)nnf)
-    NLI_DISPATCH(d.kind, C, NLI_FIELD);
+#define NLI_FIELD(KD, CC, PP) k_dist_field<KD, CC, PP><<<g, kBlock, 0, s>>>(a, (const int2 *)nnf)
+    NLI_DISPATCH(d.kind, C, d.patch, NLI_FIELD);
 #undef NLI_FIELD
     NLI_LAUNCH_CHECK();
 }

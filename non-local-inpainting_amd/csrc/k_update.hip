@@ -28,6 +28,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace nli {
@@ -360,184 +361,60 @@ __global__ void k_cg_rhs(const int *__restrict__ pts, int n, const float *__rest
     }
 }
 
+// ---------------- CG ----------------
+// Device state per channel c:
+//   hist[c * hist_len + it] = |r_it|^2, written by block 0 of phase A of iteration it
+//   done_at[c]  = the iteration count at which channel c stopped (-1 while running),
+//                 written by block 0 of the phase A that detects the stop
+//   prr, prq    = block partial sums of r.r and r.(A d) ([c * nb + block])
+// Every kernel first sums the previous kernel's partials itself, in block order, so
+// all blocks see identical scalars and no grid-wide fence or atomic is needed.
+template <typename T>
 struct CGArgs {
     const Stencil *st;
     const int4 *nbr;
-    const double *b;
-    int n, nc, nb;
-    double *x[2];
-    double *d[2];
-    double *r;
-    double *partials;
-    unsigned *counter;
-    double *scal;   // S_COUNT per channel
+    const double *b;   // right-hand side (FP64)
+    int n, nb, hist_len;
+    T *x[2];
+    T *d[2];
+    T *r;
+    double *prr, *prq;
+    double *hist;
+    int *done_at;
     float tol;
     int limit;
+    int refine_every;  // T = float: residual recomputed in FP64 every k iterations (0 = never)
 };
 
-// r0 = b - A x0, d0 = r0, nr0 = |r0|^2
 template <int NC>
-__global__ void k_cg_init(CGArgs a)
+__device__ inline void sum_partials(const double *part, int nb, double (&tot)[NC])
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[NC];
-    for (int c = 0; c < NC; c++) acc[c] = 0.0;
-    if (k < a.n) {
-        Stencil s = a.st[k];
-        int4 nb = a.nbr[k];
-        for (int c = 0; c < NC; c++) {
-            const double *xv = a.x[0] + (size_t)c * a.n;
-            double xc = xv[k];
-            double rv = a.b[(size_t)c * a.n + k] - s.a1 * xc + lap(s, nb, xv, xc);
-            a.r[(size_t)c * a.n + k] = rv;
-            a.d[0][(size_t)c * a.n + k] = rv;
-            acc[c] = rv * rv;
-        }
-    }
-    block_reduce_store<NC>(acc, a.partials, a.nb);
-    double tot[NC];
-    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
-        for (int c = 0; c < NC; c++) {
-            double *sc = a.scal + c * S_COUNT;
-            sc[S_NR0] = tot[c];
-            sc[S_NR] = tot[c];
-            sc[S_DONE] = (tot[c] > 0) ? 0.0 : 1.0;
-            sc[S_ITER] = 0.0;
-        }
-}
-
-// iteration `it`, phase A: d_it = r + beta d_{it-1} (it > 0), rdotq = <r, A d_it>, alpha
-template <int NC>
-__global__ void k_cg_a(CGArgs a, int it)
-{
-    __shared__ double sdone[NC], sbeta[NC];
-    if (threadIdx.x < NC) {
-        sdone[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_DONE];
-        sbeta[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_BETA];
+    __shared__ double sh[NC][kBlock];
+    for (int c = 0; c < NC; c++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) s += part[(size_t)c * nb + b];
+        sh[c][threadIdx.x] = s;
     }
     __syncthreads();
-    bool all_done = true;
-    for (int c = 0; c < NC; c++) all_done = all_done && sdone[c] != 0.0;
-    if (all_done) return;
-    const double *dp = a.d[it & 1];
-    double *dn = a.d[(it + 1) & 1];
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[NC];
-    for (int c = 0; c < NC; c++) acc[c] = 0.0;
-    if (k < a.n) {
-        Stencil s = a.st[k];
-        int4 nb = a.nbr[k];
-        for (int c = 0; c < NC; c++) {
-            if (sdone[c] != 0.0) continue;
-            size_t o = (size_t)c * a.n;
-            const double *rv = a.r + o, *dv = dp + o;
-            double beta = sbeta[c];
-            auto D = [&](int j) { return it == 0 ? dv[j] : rv[j] + beta * dv[j]; };
-            double dc = D(k);
-            double ut = nb.x >= 0 ? D(nb.x) : 0.0;
-            double ul = nb.y >= 0 ? D(nb.y) : 0.0;
-            double ub = nb.z >= 0 ? D(nb.z) : 0.0;
-            double ur = nb.w >= 0 ? D(nb.w) : 0.0;
-            double L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - dc * s.sum;
-            dn[o + k] = dc;
-            acc[c] = rv[k] * (s.a1 * dc - L);
-        }
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int c = 0; c < NC; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + off];
+        __syncthreads();
     }
-    block_reduce_store<NC>(acc, a.partials, a.nb);
-    double tot[NC];
-    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
-        for (int c = 0; c < NC; c++) {
-            double *sc = a.scal + c * S_COUNT;
-            if (sc[S_DONE] != 0.0) continue;
-            sc[S_RDOTQ] = tot[c];
-            sc[S_ALPHA] = sc[S_NR] / tot[c];
-        }
-}
-
-// phase B: x_{it+1} = x_it + alpha d_it, r = b - A x_{it+1}, nr, beta, stop test
-template <int NC>
-__global__ void k_cg_b(CGArgs a, int it)
-{
-    __shared__ double sdone[NC], salpha[NC];
-    if (threadIdx.x < NC) {
-        sdone[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_DONE];
-        salpha[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_ALPHA];
-    }
+    for (int c = 0; c < NC; c++) tot[c] = sh[c][0];
     __syncthreads();
-    bool all_done = true;
-    for (int c = 0; c < NC; c++) all_done = all_done && sdone[c] != 0.0;
-    if (all_done) return;
-    const double *xp = a.x[it & 1];
-    double *xn = a.x[(it + 1) & 1];
-    const double *dcur = a.d[(it + 1) & 1];
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[NC];
-    for (int c = 0; c < NC; c++) acc[c] = 0.0;
-    if (k < a.n) {
-        Stencil s = a.st[k];
-        int4 nb = a.nbr[k];
-        for (int c = 0; c < NC; c++) {
-            if (sdone[c] != 0.0) continue;
-            size_t o = (size_t)c * a.n;
-            const double *xv = xp + o, *dv = dcur + o;
-            double alpha = salpha[c];
-            auto X = [&](int j) { return xv[j] + alpha * dv[j]; };
-            double xc = X(k);
-            double ut = nb.x >= 0 ? X(nb.x) : 0.0;
-            double ul = nb.y >= 0 ? X(nb.y) : 0.0;
-            double ub = nb.z >= 0 ? X(nb.z) : 0.0;
-            double ur = nb.w >= 0 ? X(nb.w) : 0.0;
-            double L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - xc * s.sum;
-            double rv = a.b[o + k] - s.a1 * xc + L;
-            xn[o + k] = xc;
-            a.r[o + k] = rv;
-            acc[c] = rv * rv;
-        }
-    }
-    block_reduce_store<NC>(acc, a.partials, a.nb);
-    double tot[NC];
-    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
-        for (int c = 0; c < NC; c++) {
-            double *sc = a.scal + c * S_COUNT;
-            if (sc[S_DONE] != 0.0) continue;
-            double nr_old = sc[S_NR];
-            double nr = tot[c];
-            sc[S_NR] = nr;
-            sc[S_BETA] = nr / nr_old;
-            bool stop = (nr < (double)a.tol * sc[S_NR0]) || (it >= a.limit);
-            sc[S_ITER] = (double)(it + 1);
-            if (stop) sc[S_DONE] = 1.0;
-        }
 }
 
-// ---------------- mixed precision CG (FP32 vectors, FP64 dots) ----------------
-struct CGArgsF {
-    const Stencil *st;
-    const int4 *nbr;
-    const double *b;    // FP64 right-hand side (for the refinement)
-    int n, nc, nb;
-    float *x[2];
-    float *d[2];
-    float *r;
-    double *partials;
-    unsigned *counter;
-    double *scal;
-    float tol;
-    int limit;
-    int refine_every;
+template <typename T>
+struct StencilOf {
+    T at, al, ab, ar, sum, a1;
+    __device__ explicit StencilOf(const Stencil &s)
+        : at((T)s.at), al((T)s.al), ab((T)s.ab), ar((T)s.ar), sum((T)s.sum), a1((T)s.a1) {}
 };
 
-struct StencilF {
-    float at, al, ab, ar, sum, a1;
-};
-
-__device__ inline StencilF to_f(const Stencil &s)
-{
-    return StencilF{(float)s.at, (float)s.al, (float)s.ab, (float)s.ar, (float)s.sum, (float)s.a1};
-}
-
-template <int NC>
-__global__ void k_cgf_init(CGArgsF a, const double *x0)
+// r0 = b - A x0 (FP64), d0 = r0, partials of |r0|^2
+template <int NC, typename T>
+__global__ void k_cg_init(CGArgs<T> a, const double *x0)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[NC];
@@ -549,140 +426,135 @@ __global__ void k_cgf_init(CGArgsF a, const double *x0)
             size_t o = (size_t)c * a.n;
             const double *xv = x0 + o;
             double xc = xv[k];
-            double rv = a.b[o + k] - s.a1 * xc + lap(s, nb, xv, xc);   // FP64 initial residual
-            a.x[0][o + k] = (float)xc;
-            a.r[o + k] = (float)rv;
-            a.d[0][o + k] = (float)rv;
+            double rv = a.b[o + k] - s.a1 * xc + lap(s, nb, xv, xc);
+            a.x[0][o + k] = (T)xc;
+            a.r[o + k] = (T)rv;
+            a.d[0][o + k] = (T)rv;
             acc[c] = rv * rv;
         }
     }
-    block_reduce_store<NC>(acc, a.partials, a.nb);
-    double tot[NC];
-    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
-        for (int c = 0; c < NC; c++) {
-            double *sc = a.scal + c * S_COUNT;
-            sc[S_NR0] = tot[c];
-            sc[S_NR] = tot[c];
-            sc[S_DONE] = (tot[c] > 0) ? 0.0 : 1.0;
-            sc[S_ITER] = 0.0;
-        }
+    block_reduce_store<NC>(acc, a.prr, a.nb);
 }
 
-template <int NC>
-__global__ void k_cgf_a(CGArgsF a, int it)
+// phase A of iteration it: stop test on |r_it|^2, d_it = r_it + beta d_{it-1} (it > 0),
+// partials of <r_it, A d_it>
+template <int NC, typename T>
+__global__ void k_cg_a(CGArgs<T> a, int it)
 {
-    __shared__ double sdone[NC], sbeta[NC];
-    if (threadIdx.x < NC) {
-        sdone[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_DONE];
-        sbeta[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_BETA];
-    }
+    double tot[NC];
+    sum_partials<NC>(a.prr, a.nb, tot);
+    __shared__ int srun[NC];
+    __shared__ double sbeta[NC];
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NC; c++) {
+            bool run = a.done_at[c] < 0;
+            if (run) {
+                double nr = tot[c];
+                double *h = a.hist + (size_t)c * a.hist_len;
+                double nr0 = (it == 0) ? nr : h[0];
+                bool stop = (it == 0) ? !(nr0 > 0) : ((nr < (double)a.tol * nr0) || (it - 1 >= a.limit));
+                if (blockIdx.x == 0) {
+                    h[it] = nr;
+                    if (stop) a.done_at[c] = it;
+                }
+                sbeta[c] = (it == 0 || stop) ? 0.0 : nr / h[it - 1];
+                run = !stop;
+            }
+            srun[c] = run;
+        }
     __syncthreads();
-    bool all_done = true;
-    for (int c = 0; c < NC; c++) all_done = all_done && sdone[c] != 0.0;
-    if (all_done) return;
-    const float *dp = a.d[it & 1];
-    float *dn = a.d[(it + 1) & 1];
+    bool any = false;
+    for (int c = 0; c < NC; c++) any = any || srun[c];
+    if (!any) return;
+    const T *dp = a.d[it & 1];
+    T *dn = a.d[(it + 1) & 1];
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[NC];
     for (int c = 0; c < NC; c++) acc[c] = 0.0;
     if (k < a.n) {
-        StencilF s = to_f(a.st[k]);
+        StencilOf<T> s(a.st[k]);
         int4 nb = a.nbr[k];
         for (int c = 0; c < NC; c++) {
-            if (sdone[c] != 0.0) continue;
+            if (!srun[c]) continue;
             size_t o = (size_t)c * a.n;
-            const float *rv = a.r + o, *dv = dp + o;
-            float beta = (float)sbeta[c];
+            const T *rv = a.r + o, *dv = dp + o;
+            T beta = (T)sbeta[c];
             auto D = [&](int j) { return it == 0 ? dv[j] : rv[j] + beta * dv[j]; };
-            float dc = D(k);
-            float ut = nb.x >= 0 ? D(nb.x) : 0.0f;
-            float ul = nb.y >= 0 ? D(nb.y) : 0.0f;
-            float ub = nb.z >= 0 ? D(nb.z) : 0.0f;
-            float ur = nb.w >= 0 ? D(nb.w) : 0.0f;
-            float L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - dc * s.sum;
+            T dc = D(k);
+            T ut = nb.x >= 0 ? D(nb.x) : (T)0;
+            T ul = nb.y >= 0 ? D(nb.y) : (T)0;
+            T ub = nb.z >= 0 ? D(nb.z) : (T)0;
+            T ur = nb.w >= 0 ? D(nb.w) : (T)0;
+            T L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - dc * s.sum;
             dn[o + k] = dc;
             acc[c] = (double)rv[k] * (double)(s.a1 * dc - L);
         }
     }
-    block_reduce_store<NC>(acc, a.partials, a.nb);
-    double tot[NC];
-    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
-        for (int c = 0; c < NC; c++) {
-            double *sc = a.scal + c * S_COUNT;
-            if (sc[S_DONE] != 0.0) continue;
-            sc[S_RDOTQ] = tot[c];
-            sc[S_ALPHA] = sc[S_NR] / tot[c];
-        }
+    block_reduce_store<NC>(acc, a.prq, a.nb);
 }
 
-template <int NC>
-__global__ void k_cgf_b(CGArgsF a, int it)
+// phase B of iteration it: alpha = |r_it|^2 / <r_it, A d_it>, x_{it+1} = x_it + alpha d_it,
+// r_{it+1} = b - A x_{it+1}, partials of |r_{it+1}|^2
+template <int NC, typename T>
+__global__ void k_cg_b(CGArgs<T> a, int it)
 {
-    __shared__ double sdone[NC], salpha[NC];
-    if (threadIdx.x < NC) {
-        sdone[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_DONE];
-        salpha[threadIdx.x] = a.scal[threadIdx.x * S_COUNT + S_ALPHA];
-    }
+    double tot[NC];
+    sum_partials<NC>(a.prq, a.nb, tot);
+    __shared__ int srun[NC];
+    __shared__ double salpha[NC];
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NC; c++) {
+            bool run = a.done_at[c] < 0;
+            srun[c] = run;
+            salpha[c] = run ? a.hist[(size_t)c * a.hist_len + it] / tot[c] : 0.0;
+        }
     __syncthreads();
-    bool all_done = true;
-    for (int c = 0; c < NC; c++) all_done = all_done && sdone[c] != 0.0;
-    if (all_done) return;
-    const float *xp = a.x[it & 1];
-    float *xn = a.x[(it + 1) & 1];
-    const float *dcur = a.d[(it + 1) & 1];
-    const bool refine = a.refine_every > 0 && ((it + 1) % a.refine_every == 0);
+    bool any = false;
+    for (int c = 0; c < NC; c++) any = any || srun[c];
+    if (!any) return;
+    const T *xp = a.x[it & 1];
+    T *xn = a.x[(it + 1) & 1];
+    const T *dcur = a.d[(it + 1) & 1];
+    const bool refine = std::is_same<T, float>::value && a.refine_every > 0 && ((it + 1) % a.refine_every == 0);
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[NC];
     for (int c = 0; c < NC; c++) acc[c] = 0.0;
     if (k < a.n) {
         Stencil s64 = a.st[k];
-        StencilF s = to_f(s64);
+        StencilOf<T> s(s64);
         int4 nb = a.nbr[k];
         for (int c = 0; c < NC; c++) {
-            if (sdone[c] != 0.0) continue;
+            if (!srun[c]) continue;
             size_t o = (size_t)c * a.n;
-            const float *xv = xp + o, *dv = dcur + o;
-            float alpha = (float)salpha[c];
+            const T *xv = xp + o, *dv = dcur + o;
+            T alpha = (T)salpha[c];
             auto X = [&](int j) { return xv[j] + alpha * dv[j]; };
-            float xc = X(k);
-            float ut = nb.x >= 0 ? X(nb.x) : 0.0f;
-            float ul = nb.y >= 0 ? X(nb.y) : 0.0f;
-            float ub = nb.z >= 0 ? X(nb.z) : 0.0f;
-            float ur = nb.w >= 0 ? X(nb.w) : 0.0f;
+            T xc = X(k);
+            T ut = nb.x >= 0 ? X(nb.x) : (T)0;
+            T ul = nb.y >= 0 ? X(nb.y) : (T)0;
+            T ub = nb.z >= 0 ? X(nb.z) : (T)0;
+            T ur = nb.w >= 0 ? X(nb.w) : (T)0;
             double rv;
             if (refine) {   // residual replacement: b - A x evaluated in FP64
                 double L = (double)ut * s64.at + (double)ul * s64.al + (double)ub * s64.ab + (double)ur * s64.ar -
                            (double)xc * s64.sum;
                 rv = a.b[o + k] - s64.a1 * (double)xc + L;
             } else {
-                float L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - xc * s.sum;
-                rv = (double)((float)a.b[o + k] - s.a1 * xc + L);
+                T L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - xc * s.sum;
+                rv = (double)((T)a.b[o + k] - s.a1 * xc + L);
             }
             xn[o + k] = xc;
-            a.r[o + k] = (float)rv;
+            a.r[o + k] = (T)rv;
             acc[c] = rv * rv;
         }
     }
-    block_reduce_store<NC>(acc, a.partials, a.nb);
-    double tot[NC];
-    if (last_block_total<NC>(a.partials, a.nb, a.counter, tot) && threadIdx.x == 0)
-        for (int c = 0; c < NC; c++) {
-            double *sc = a.scal + c * S_COUNT;
-            if (sc[S_DONE] != 0.0) continue;
-            double nr_old = sc[S_NR];
-            double nr = tot[c];
-            sc[S_NR] = nr;
-            sc[S_BETA] = nr / nr_old;
-            bool stop = (nr < (double)a.tol * sc[S_NR0]) || (it >= a.limit);
-            sc[S_ITER] = (double)(it + 1);
-            if (stop) sc[S_DONE] = 1.0;
-        }
+    block_reduce_store<NC>(acc, a.prr, a.nb);
 }
 
 // final colours: image = float(u0 + x_final) on hole points; sum of (prev - new)^2
 template <typename T>
 __global__ void k_poisson_write(float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0, int nc,
-                                const int *__restrict__ pts, int n, T *x0, T *x1, const double *__restrict__ scal,
+                                const int *__restrict__ pts, int n, T *x0, T *x1, const int *__restrict__ done_at,
                                 double *__restrict__ partials, unsigned *counter, double *result)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -690,8 +562,7 @@ __global__ void k_poisson_write(float *__restrict__ img, const float *__restrict
     if (k < n) {
         size_t p = (size_t)pts[k];
         for (int ch = 0; ch < nc; ch++) {
-            int it = (int)scal[ch * S_COUNT + S_ITER];
-            const T *xf = (it & 1) ? x1 : x0;
+            const T *xf = (done_at[ch] & 1) ? x1 : x0;
             double u0 = (double)orig[p * Cs + c0 + ch];
             float v = (float)(u0 + (double)xf[(size_t)ch * n + k]);
             float prev = img[p * Cs + c0 + ch];
@@ -725,104 +596,72 @@ void upload_weights(UpdWork &w, int P, float sigma, hipStream_t s)
     NLI_HIP(hipStreamSynchronize(s));
 }
 
-// scalar/partials/counter layout in w.scalars (doubles): [0, S_COUNT*4) scalars,
-// then one slot used as the result; counters live in w.flag (as unsigned).
-template <int NC>
-void run_cg(const CGParams &cg, int n, UpdWork &w, double *x0, hipStream_t s, CGStats *stats, const Stencil *st,
-            const int4 *nbr, int &final_parity_unused)
+// Runs the CG on n unknowns x NC channels; returns with x in a.x[done_at & 1].
+template <int NC, typename T>
+void run_cg(const CGParams &cg, int n, UpdWork &w, CGArgs<T> a, hipStream_t s, CGStats *stats)
 {
-    (void)final_parity_unused;
     const int nb = (int)grid_for(n);
-    unsigned *counter = (unsigned *)w.flag.p;
-    double *scal = w.scalars.p;
-    if (cg.precision == 0) {
-        CGArgs a;
-        a.st = st;
-        a.nbr = nbr;
-        a.b = w.b.p;
-        a.n = n;
-        a.nc = NC;
-        a.nb = nb;
-        a.x[0] = x0;
-        a.x[1] = w.x1.p;
-        a.d[0] = w.d0.p;
-        a.d[1] = w.d1.p;
-        a.r = w.r.p;
-        a.partials = w.partials.p;
-        a.counter = counter;
-        a.scal = scal;
-        a.tol = cg.tol;
-        a.limit = cg.limit;
-        k_cg_init<NC><<<nb, kBlock, 0, s>>>(a);
+    k_cg_init<NC, T><<<nb, kBlock, 0, s>>>(a, w.x0.p);
+    NLI_LAUNCH_CHECK();
+    const int poll = 8;
+    // algorithmic bytes of one iteration of the reference recurrence (two matvecs):
+    // 2 x 13 words per unknown and channel
+    const double it_bytes = 2.0 * 13 * sizeof(T) * NC * (double)n;
+    for (int it = 0; it <= cg.limit + 1; it++) {
+        prof::begin(TK_CG, s);
+        k_cg_a<NC, T><<<nb, kBlock, 0, s>>>(a, it);
+        if (it <= cg.limit) k_cg_b<NC, T><<<nb, kBlock, 0, s>>>(a, it);
         NLI_LAUNCH_CHECK();
-        const int poll = 8;
-        // algorithmic bytes of one iteration of the reference recurrence (two matvecs):
-        // 2 x 13 words x 8 bytes per unknown and channel
-        const double it_bytes = 2.0 * 13 * 8 * NC * (double)n;
-        for (int it = 0; it <= cg.limit; it++) {
-            prof::begin(TK_CG, s);
-            k_cg_a<NC><<<nb, kBlock, 0, s>>>(a, it);
-            k_cg_b<NC><<<nb, kBlock, 0, s>>>(a, it);
-            NLI_LAUNCH_CHECK();
-            prof::end(TK_CG, s, it_bytes, (double)NC * n);
-            if ((it + 1) % poll == 0 && it < cg.limit) {
-                double flags[kMaxNC * S_COUNT];
-                NLI_HIP(hipMemcpyAsync(flags, scal, sizeof(double) * NC * S_COUNT, hipMemcpyDeviceToHost, s));
-                NLI_HIP(hipStreamSynchronize(s));
-                bool all = true;
-                for (int c = 0; c < NC; c++) all = all && flags[c * S_COUNT + S_DONE] != 0.0;
-                if (all) break;
-            }
-        }
-    } else {
-        CGArgsF a;
-        a.st = st;
-        a.nbr = nbr;
-        a.b = w.b.p;
-        a.n = n;
-        a.nc = NC;
-        a.nb = nb;
-        a.x[0] = w.xs0.p;
-        a.x[1] = w.xs1.p;
-        a.d[0] = w.ds0.p;
-        a.d[1] = w.ds1.p;
-        a.r = w.rs.p;
-        a.partials = w.partials.p;
-        a.counter = counter;
-        a.scal = scal;
-        a.tol = cg.tol;
-        a.limit = cg.limit;
-        a.refine_every = cg.refine_every;
-        k_cgf_init<NC><<<nb, kBlock, 0, s>>>(a, x0);
-        NLI_LAUNCH_CHECK();
-        const int poll = 8;
-        const double it_bytes = 2.0 * 13 * 4 * NC * (double)n;   // FP32 vectors
-        for (int it = 0; it <= cg.limit; it++) {
-            prof::begin(TK_CG, s);
-            k_cgf_a<NC><<<nb, kBlock, 0, s>>>(a, it);
-            k_cgf_b<NC><<<nb, kBlock, 0, s>>>(a, it);
-            NLI_LAUNCH_CHECK();
-            prof::end(TK_CG, s, it_bytes, (double)NC * n);
-            if ((it + 1) % poll == 0 && it < cg.limit) {
-                double flags[kMaxNC * S_COUNT];
-                NLI_HIP(hipMemcpyAsync(flags, scal, sizeof(double) * NC * S_COUNT, hipMemcpyDeviceToHost, s));
-                NLI_HIP(hipStreamSynchronize(s));
-                bool all = true;
-                for (int c = 0; c < NC; c++) all = all && flags[c * S_COUNT + S_DONE] != 0.0;
-                if (all) break;
-            }
+        prof::end(TK_CG, s, it_bytes, (double)NC * n);
+        if ((it + 1) % poll == 0 && it <= cg.limit) {
+            int done[kMaxNC];
+            NLI_HIP(hipMemcpyAsync(done, a.done_at, sizeof(int) * NC, hipMemcpyDeviceToHost, s));
+            NLI_HIP(hipStreamSynchronize(s));
+            bool all = true;
+            for (int c = 0; c < NC; c++) all = all && done[c] >= 0;
+            if (all) break;
         }
     }
     if (stats) {
-        double h[kMaxNC * S_COUNT];
-        NLI_HIP(hipMemcpyAsync(h, scal, sizeof(double) * NC * S_COUNT, hipMemcpyDeviceToHost, s));
+        int done[kMaxNC];
+        NLI_HIP(hipMemcpyAsync(done, a.done_at, sizeof(int) * NC, hipMemcpyDeviceToHost, s));
         NLI_HIP(hipStreamSynchronize(s));
         for (int c = 0; c < NC; c++) {
-            stats->iterations[c] = (int)h[c * S_COUNT + S_ITER];
-            stats->nr0[c] = h[c * S_COUNT + S_NR0];
-            stats->nr[c] = h[c * S_COUNT + S_NR];
+            stats->iterations[c] = done[c];
+            NLI_HIP(hipMemcpyAsync(&stats->nr0[c], a.hist + (size_t)c * a.hist_len, sizeof(double),
+                                   hipMemcpyDeviceToHost, s));
+            NLI_HIP(hipMemcpyAsync(&stats->nr[c], a.hist + (size_t)c * a.hist_len + std::max(done[c], 0),
+                                   sizeof(double), hipMemcpyDeviceToHost, s));
         }
+        NLI_HIP(hipStreamSynchronize(s));
     }
+}
+
+template <typename T>
+CGArgs<T> cg_args(const CGParams &cg, int n, int nc, UpdWork &w, const Stencil *st, const int4 *nbr, T *x0, T *x1,
+                  T *d0, T *d1, T *r)
+{
+    CGArgs<T> a;
+    a.st = st;
+    a.nbr = nbr;
+    a.b = w.b.p;
+    a.n = n;
+    a.nb = (int)grid_for(n);
+    a.hist_len = cg.limit + 2;
+    a.x[0] = x0;
+    a.x[1] = x1;
+    a.d[0] = d0;
+    a.d[1] = d1;
+    a.r = r;
+    a.prr = w.partials.p;
+    a.prq = w.partials.p + (size_t)kMaxNC * a.nb;
+    a.hist = w.scalars.p + kMaxNC * S_COUNT + 8;
+    a.done_at = w.flag.p + 2;
+    a.tol = cg.tol;
+    a.limit = cg.limit;
+    a.refine_every = cg.refine_every;
+    (void)nc;
+    return a;
 }
 
 double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *image, const float *orig, int Cs,
@@ -848,10 +687,11 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     k_divergence<<<g2, kBlock, 0, s>>>(ext, w.F1.p, w.F2.p, nc, W, H, lambda != 1 ? 1 : 0, w.f2.p);
     NLI_LAUNCH_CHECK();
     const int nb = (int)grid_for(std::max(n, 1));
-    w.scalars.ensure(kMaxNC * S_COUNT + 8);
-    w.partials.ensure((size_t)kMaxNC * nb + 8);
-    w.flag.ensure(4);
-    NLI_HIP(hipMemsetAsync(w.flag.p, 0, 4 * sizeof(int), s));
+    w.scalars.ensure(kMaxNC * S_COUNT + 8 + (size_t)kMaxNC * (cg.limit + 2));
+    w.partials.ensure((size_t)2 * kMaxNC * nb + 8);
+    w.flag.ensure(2 + kMaxNC + 2);
+    NLI_HIP(hipMemsetAsync(w.flag.p, 0, 2 * sizeof(int), s));
+    fill_i32(w.flag.p + 2, -1, kMaxNC, s);
     double *result = w.scalars.p + kMaxNC * S_COUNT;
     if (n == 0) {
         if (stats)
@@ -885,21 +725,31 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
         w.ds1.ensure((size_t)n * nc);
         w.rs.ensure((size_t)n * nc);
     }
-    int unused = 0;
-    switch (nc) {
-    case 1: run_cg<1>(cg, n, w, w.x0.p, s, stats, st, nbr, unused); break;
-    case 2: run_cg<2>(cg, n, w, w.x0.p, s, stats, st, nbr, unused); break;
-    case 3: run_cg<3>(cg, n, w, w.x0.p, s, stats, st, nbr, unused); break;
-    case 4: run_cg<4>(cg, n, w, w.x0.p, s, stats, st, nbr, unused); break;
-    default: throw std::invalid_argument("poisson: 1 to 4 channels");
-    }
     unsigned *counter = (unsigned *)w.flag.p + 1;
-    if (cg.precision == 0)
-        k_poisson_write<double><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.x0.p, w.x1.p,
-                                                      w.scalars.p, w.partials.p, counter, result);
-    else
-        k_poisson_write<float><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.xs0.p, w.xs1.p,
-                                                     w.scalars.p, w.partials.p, counter, result);
+    const int *done_at = w.flag.p + 2;
+    if (cg.precision == 0) {
+        CGArgs<double> a = cg_args<double>(cg, n, nc, w, st, nbr, w.x0.p, w.x1.p, w.d0.p, w.d1.p, w.r.p);
+        switch (nc) {
+        case 1: run_cg<1, double>(cg, n, w, a, s, stats); break;
+        case 2: run_cg<2, double>(cg, n, w, a, s, stats); break;
+        case 3: run_cg<3, double>(cg, n, w, a, s, stats); break;
+        case 4: run_cg<4, double>(cg, n, w, a, s, stats); break;
+        default: throw std::invalid_argument("poisson: 1 to 4 channels");
+        }
+        k_poisson_write<double><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.x0.p, w.x1.p, done_at,
+                                                      w.partials.p, counter, result);
+    } else {
+        CGArgs<float> a = cg_args<float>(cg, n, nc, w, st, nbr, w.xs0.p, w.xs1.p, w.ds0.p, w.ds1.p, w.rs.p);
+        switch (nc) {
+        case 1: run_cg<1, float>(cg, n, w, a, s, stats); break;
+        case 2: run_cg<2, float>(cg, n, w, a, s, stats); break;
+        case 3: run_cg<3, float>(cg, n, w, a, s, stats); break;
+        case 4: run_cg<4, float>(cg, n, w, a, s, stats); break;
+        default: throw std::invalid_argument("poisson: 1 to 4 channels");
+        }
+        k_poisson_write<float><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.xs0.p, w.xs1.p, done_at,
+                                                     w.partials.p, counter, result);
+    }
     NLI_LAUNCH_CHECK();
     return read_scalar(result, s);
 }
