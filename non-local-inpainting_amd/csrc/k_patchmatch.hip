@@ -20,14 +20,15 @@
 // it cannot be accepted; the accept/reject decision is then identical to the full
 // evaluation (all terms are non-negative and rounding is monotone).
 //
-// Work layout.  One thread per target pixel, 32x8 target tiles (one 256-thread block
-// each; only tiles holding a target are launched).  The block stages the target
+// Work layout.  16x4 target tiles, one 256-thread block each (4 lanes per target in a
+// sweep, 1 in the initialisation; only tiles holding a target are launched).  The block stages the target
 // patch data of its tile plus the patch halo in LDS as planar float planes (colour
 // channels, then the 2C gradient planes); source patches are gathered from global
 // memory (L2 / MALL resident), row by row.
 #include "nli_internal.h"
 
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -35,8 +36,11 @@ namespace nli {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int TX = 32;
-constexpr int TY = 8;
+constexpr int TX = 16;     // target tile: 16 x 4 targets
+constexpr int TY = 4;
+constexpr int kLanes = 4;  // sweep lanes per target (kBlock = TX * TY * kLanes)
+constexpr int kXCDs = 8;
+static_assert(kBlock == TX * TY * kLanes, "one sweep block covers one tile");
 
 // ---- counter-based RNG (SplitMix64 finaliser over a hashed counter) ----
 __device__ inline uint64_t mix64(uint64_t z)
@@ -75,7 +79,7 @@ struct PMArgs {
     int2 *nnf_out;
     float *dist_out;
     const int *tiles;
-    int ntiles_x;
+    int ntiles_x, ntiles;
     int maxw, shots, step;
     uint64_t seed;
     uint32_t stream;
@@ -117,11 +121,110 @@ struct TgtGlobal {
     }
 };
 
+// Gradient distances (L2Combined with C = 1 or 3, L2RGBD) for a compile-time patch
+// side: the source gradient is formed on the fly from the image rows the colour term
+// loads anyway (forward differences, zero past the last column/row, the exact float
+// subtractions of gradient.cpp:32-73), so a source patch costs (P+1)^2 C floats of
+// gathers instead of 3 P^2 C.  Arithmetic and summation order are those of
+// patch_distance below, so results are identical.
+template <int KIND, int C, int PS, class T>
+__device__ float patch_distance_fly(const PMArgs &a, const float *__restrict__ w, int sx, int sy, const T &t,
+                                    float bd)
+{
+    constexpr int P = PS, r = PS / 2;
+    constexpr unsigned PP = (unsigned)(P * P);
+    const double bound = (double)bd * (double)PP * (1.0 + 1e-6);
+    const double lam = (double)a.lambda, lam1 = (double)(1 - a.lambda);
+    const double lamd = (double)a.lambda_d, lamd1 = (double)(1 - a.lambda_d);
+    double dist = 0.0, gdist = 0.0, dist_d = 0.0, gdist_d = 0.0;
+    const int x0 = sx - r;
+    const bool has_right = sx + r + 1 < a.sW;
+    float cur[P + 1][C], nxt[P + 1][C];
+    {
+        const float *row = a.src + ((size_t)(sy - r) * a.sW + x0) * C;
+#pragma unroll
+        for (int dx = 0; dx < P; dx++)
+#pragma unroll
+            for (int c = 0; c < C; c++) cur[dx][c] = row[dx * C + c];
+#pragma unroll
+        for (int c = 0; c < C; c++) cur[P][c] = has_right ? row[P * C + c] : 0.0f;
+    }
+    for (int dy = 0; dy < P; dy++) {
+        const int ys = sy - r + dy;
+        const bool has_down = ys + 1 < a.sH;
+        if (has_down) {
+            const float *row = a.src + ((size_t)(ys + 1) * a.sW + x0) * C;
+#pragma unroll
+            for (int dx = 0; dx < P; dx++)
+#pragma unroll
+                for (int c = 0; c < C; c++) nxt[dx][c] = row[dx * C + c];
+#pragma unroll
+            for (int c = 0; c < C; c++) nxt[P][c] = has_right ? row[P * C + c] : 0.0f;
+        }
+#pragma unroll
+        for (int dx = 0; dx < P; dx++) {
+            const float wt = w[dy * P + dx];
+            float g[2 * C];
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                g[2 * c] = (dx == P - 1 && !has_right) ? 0.0f : cur[dx + 1][c] - cur[dx][c];
+                g[2 * c + 1] = has_down ? nxt[dx][c] - cur[dx][c] : 0.0f;
+            }
+            if constexpr (KIND == DIST_L2COMBINED) {
+                float norm = 0.0f, gnorm = 0.0f;
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    float d = cur[dx][c] - t.v(c, dx, dy);
+                    norm = (c == 0) ? d * d : norm + d * d;
+                }
+#pragma unroll
+                for (int k = 0; k < 2 * C; k++) {
+                    float d = g[k] - t.v(C + k, dx, dy);
+                    gnorm = (k == 0) ? d * d : gnorm + d * d;
+                }
+                dist += (double)(wt * norm);
+                gdist += (double)(wt * gnorm);
+            } else {   // DIST_L2RGBD, C == 4
+                double norm = 0.0, norm_d = 0.0, gnorm = 0.0, gnorm_d = 0.0;
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    float d = cur[dx][c] - t.v(c, dx, dy);
+                    float gx = g[2 * c] - t.v(C + 2 * c, dx, dy);
+                    float gy = g[2 * c + 1] - t.v(C + 2 * c + 1, dx, dy);
+                    if (c == 3) {
+                        norm_d += (double)(d * d);
+                        gnorm_d += (double)(gx * gx + gy * gy);
+                    } else {
+                        norm += (double)(d * d);
+                        gnorm += (double)(gx * gx + gy * gy);
+                    }
+                }
+                dist += (double)wt * norm;
+                dist_d += (double)wt * norm_d;
+                gdist += (double)wt * gnorm;
+                gdist_d += (double)wt * gnorm_d;
+            }
+        }
+        double partial = lam * dist + lam1 * gdist;
+        if constexpr (KIND == DIST_L2RGBD) partial = partial + lamd * dist_d + lamd1 * gdist_d;
+        if (partial > bound) return INFINITY;
+#pragma unroll
+        for (int dx = 0; dx <= P; dx++)
+#pragma unroll
+            for (int c = 0; c < C; c++) cur[dx][c] = nxt[dx][c];
+    }
+    double total = lam * dist + lam1 * gdist;
+    if constexpr (KIND == DIST_L2RGBD) total = total + lamd * dist_d + lamd1 * gdist_d;
+    return (float)(total / (double)PP);
+}
+
 // Patch distance between the source patch centred at (sx, sy) and the target patch of
 // `t`.  Returns +inf as soon as the partial sum proves the result is >= bd.
 template <int KIND, int C, int PS, class T>
 __device__ float patch_distance(const PMArgs &a, const float *__restrict__ w, int sx, int sy, const T &t, float bd)
 {
+    if constexpr (PS > 0 && (KIND == DIST_L2RGBD || (KIND == DIST_L2COMBINED && (C == 1 || C == 3))))
+        return patch_distance_fly<KIND, C, PS>(a, w, sx, sy, t, bd);
     const int P = PS ? PS : a.P, r = P / 2;
     const unsigned PP = (unsigned)(P * P);
     if constexpr (KIND == DIST_L2 || KIND == DIST_L1) {
@@ -279,7 +382,31 @@ __global__ __launch_bounds__(kBlock) void k_pm_init(PMArgs a)
     }
 }
 
-// One sweep: jump-flood propagation at distance `step` + random search.
+// (distance, candidate, order) minimum over the 4 lanes of a target group: smallest
+// distance, ties to the earliest candidate -- the outcome of evaluating the candidates
+// one after the other with a strict "<".
+__device__ inline void group_min(float &d, int2 &c, int &order)
+{
+#pragma unroll
+    for (int m = 1; m <= 2; m <<= 1) {
+        float od = __shfl_xor(d, m, kLanes);
+        int ox = __shfl_xor(c.x, m, kLanes);
+        int oy = __shfl_xor(c.y, m, kLanes);
+        int oo = __shfl_xor(order, m, kLanes);
+        if (od < d || (od == d && oo < order)) {
+            d = od;
+            c = make_int2(ox, oy);
+            order = oo;
+        }
+    }
+}
+
+// One sweep: jump-flood propagation at distance `step` + random search, 4 lanes per
+// target.  Phase 1: lane k evaluates the shifted entry of neighbour k (left, right,
+// up, down at distance `step`).  Phase 2: the search windows, centred on the phase-1
+// best, are dealt round-robin to the 4 lanes.  A lane stops a distance evaluation
+// early against its own running best, which is never below the sequential running
+// best at that candidate, so every accept/reject decision is the sequential one.
 template <int KIND, int C, int PS>
 __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
 {
@@ -290,7 +417,12 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
     const int LW = TX + 2 * r, LH = TY + 2 * r;
     const int plane = LW * LH;
     float *wl = lds + F * plane;
-    const int tile = a.tiles[blockIdx.x];
+    // blocks are dealt to the 8 XCDs round-robin: give each XCD a contiguous band of
+    // tiles so that neighbouring targets' candidates share its L2
+    const int per = (int)(gridDim.x / kXCDs);
+    const int slot = (int)(blockIdx.x % kXCDs) * per + (int)(blockIdx.x / kXCDs);
+    if (slot >= a.ntiles) return;
+    const int tile = a.tiles[slot];
     const int x0 = (tile % a.ntiles_x) * TX, y0 = (tile / a.ntiles_x) * TY;
     for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = a.w[i];
     for (int i = threadIdx.x; i < plane; i += blockDim.x) {
@@ -305,57 +437,93 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
         }
     }
     __syncthreads();
-    const int lx = (int)threadIdx.x % TX, ly = (int)threadIdx.x / TX;
+    const int lane = (int)threadIdx.x & (kLanes - 1);
+    const int tl = (int)threadIdx.x / kLanes;
+    const int lx = tl % TX, ly = tl / TX;
     const int x = x0 + lx, y = y0 + ly;
+    // the 4 lanes of a group share the target, so groups leave together
     if (x >= a.tW || y >= a.tH || !is_target(a, x, y)) return;
     const size_t p = (size_t)a.tW * y + x;
     const TgtLds t{lds, plane, LW, lx, ly};
     int2 best = a.nnf_in[p];
     float bd = a.dist_in[p];
 
-    const int s = a.step;
-    const int ox[4] = {-s, s, 0, 0};
-    const int oy[4] = {0, 0, -s, s};
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        int qx = x + ox[k], qy = y + oy[k];
-        if (!is_target(a, qx, qy)) continue;
-        int2 c = a.nnf_in[(size_t)a.tW * qy + qx];
-        c.x -= ox[k];
-        c.y -= oy[k];
-        if (!is_source(a, c.x, c.y)) continue;
-        float d = patch_distance<KIND, C, PS>(a, wl, c.x, c.y, t, bd);
+    // phase 1: propagation
+    {
+        const int s = a.step;
+        const int ox = lane == 0 ? -s : (lane == 1 ? s : 0);
+        const int oy = lane == 2 ? -s : (lane == 3 ? s : 0);
+        float d = INFINITY;
+        int2 c = best;
+        int order = INT_MAX;
+        const int qx = x + ox, qy = y + oy;
+        if (is_target(a, qx, qy)) {
+            int2 e = a.nnf_in[(size_t)a.tW * qy + qx];
+            e.x -= ox;
+            e.y -= oy;
+            if (is_source(a, e.x, e.y)) {
+                d = patch_distance<KIND, C, PS>(a, wl, e.x, e.y, t, bd);
+                c = e;
+                order = lane;
+            }
+        }
+        group_min(d, c, order);
         if (d < bd) {
             bd = d;
             best = c;
         }
     }
 
-    Rng rng = make_rng(a.seed, a.stream, (uint32_t)p);
-    const int scx = best.x, scy = best.y;
-    for (int ws = a.maxw; ws >= 1; ws /= 2) {
-        int x_min = max(scx - ws, 0), y_min = max(scy - ws, 0);
-        int x_max = min(scx + ws + 1, a.sW), y_max = min(scy + ws + 1, a.sH);
-        int cx = -1, cy = -1;
-        bool found = false;
-        for (int k = 0; k < a.shots; k++) {
-            cx = x_min + rng.below(x_max - x_min);
-            cy = y_min + rng.below(y_max - y_min);
-            if (is_source(a, cx, cy)) {
-                found = true;
-                break;
+    // phase 2: random search around the phase-1 best, in rounds of one window per lane;
+    // after each round the group's best becomes every lane's pruning bound
+    {
+        Rng rng = make_rng(a.seed, a.stream * kLanes + (uint32_t)lane, (uint32_t)p);
+        const int scx = best.x, scy = best.y;
+        int R = 0;
+        for (int ws = a.maxw; ws >= 1; ws /= 2) R++;
+        float gd = INFINITY;   // group best of phase 2 so far
+        int2 gc = best;
+        int gorder = INT_MAX;
+        for (int base = 0; base < R; base += kLanes) {
+            const int wi = base + lane;
+            float d = INFINITY;
+            int2 c = best;
+            int order = INT_MAX;
+            if (wi < R) {
+                const int ws = a.maxw >> wi;
+                int x_min = max(scx - ws, 0), y_min = max(scy - ws, 0);
+                int x_max = min(scx + ws + 1, a.sW), y_max = min(scy + ws + 1, a.sH);
+                for (int k = 0; k < a.shots; k++) {
+                    int cx = x_min + rng.below(x_max - x_min);
+                    int cy = y_min + rng.below(y_max - y_min);
+                    if (is_source(a, cx, cy)) {
+                        const float bound = fminf(bd, gd);
+                        float dd = patch_distance<KIND, C, PS>(a, wl, cx, cy, t, bound);
+                        if (dd < bound) {
+                            d = dd;
+                            c = make_int2(cx, cy);
+                            order = wi;
+                        }
+                        break;
+                    }
+                }
+            }
+            group_min(d, c, order);
+            if (d < gd || (d == gd && order < gorder)) {
+                gd = d;
+                gc = c;
+                gorder = order;
             }
         }
-        if (found) {
-            float d = patch_distance<KIND, C, PS>(a, wl, cx, cy, t, bd);
-            if (d < bd) {
-                bd = d;
-                best = make_int2(cx, cy);
-            }
+        if (gd < bd) {
+            bd = gd;
+            best = gc;
         }
     }
-    a.nnf_out[p] = best;
-    a.dist_out[p] = bd;
+    if (lane == 0) {
+        a.nnf_out[p] = best;
+        a.dist_out[p] = bd;
+    }
 }
 
 template <int KIND, int C, int PS>
@@ -500,6 +668,7 @@ Prepared prepare(const DistDesc &d, const float *src, const uint8_t *smask, int 
         }
         a.tiles = w.tiles.p;
         a.ntiles_x = ntx;
+        a.ntiles = pr.ntiles;
     }
     return pr;
 }
@@ -558,7 +727,7 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     a.dist_out = d_cur;
     a.stream = 0xFFFFu;
     prof::begin(TK_PM_INIT, s);
-#define NLI_INIT(KD, CC, PP) k_pm_init<KD, CC, PP><<<pr.ntiles, kBlock, 0, s>>>(a)
+#define NLI_INIT(KD, CC, PP) k_pm_init<KD, CC, PP><<<pr.ntiles, TX * TY, 0, s>>>(a)
     NLI_DISPATCH(d.kind, C, d.patch, NLI_INIT);
 #undef NLI_INIT
     NLI_LAUNCH_CHECK();
@@ -566,6 +735,7 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     const int r = d.patch / 2;
     const int F = (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) ? 3 * C : C;
     const size_t lds = ((size_t)F * (TX + 2 * r) * (TY + 2 * r) + (size_t)d.patch * d.patch) * sizeof(float);
+    const int sweep_blocks = (pr.ntiles + kXCDs - 1) / kXCDs * kXCDs;
     const int s0 = cfg.first_step > 0 ? cfg.first_step : auto_first_step(tW, tH);
     int levels = 0;
     while ((s0 >> levels) > 1) levels++;
@@ -583,7 +753,7 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
         a.step = s0 >> (k % (levels + 1));
         a.stream = (uint32_t)k;
         prof::begin(TK_PM_SWEEP, s);
-#define NLI_SWEEP(KD, CC, PP) k_pm_sweep<KD, CC, PP><<<pr.ntiles, kBlock, lds, s>>>(a)
+#define NLI_SWEEP(KD, CC, PP) k_pm_sweep<KD, CC, PP><<<sweep_blocks, kBlock, lds, s>>>(a)
         NLI_DISPATCH(d.kind, C, d.patch, NLI_SWEEP);
 #undef NLI_SWEEP
         NLI_LAUNCH_CHECK();
