@@ -1,13 +1,14 @@
 #!/usr/bin/env python3
 """bench.py -- inpainted MPix/s of the 1024x1024 nlpoisson workload on one MI355X.
 
-One step = one complete multiscale inpainting (ImageInpainting::process: 11
-scales, every inpainting iteration until the stopping rule, each a PatchMatch
-NNF search plus the NL-Poisson update with its CG solve) of one 1024x1024 RGB
-float32 image with a 15 % union-of-ellipses hole; lambda 0.05, patch 9, <= 200
-iterations per scale, CG tolerance 1e-4 with <= 100 iterations.  Inputs are
-resident in device memory before the timed region (device-resident driver,
-nli_inpaint_dev).  Prints one JSON line.
+One step = complete multiscale inpaintings (ImageInpainting::process: 11 scales,
+every inpainting iteration until the stopping rule, each a PatchMatch NNF search
+plus the NL-Poisson update with its CG solve) of a batch of 1024x1024 RGB float32
+images (distinct seeded textures, 15 % union-of-ellipses holes); lambda 0.05,
+patch 9, <= 200 iterations per scale, CG tolerance 1e-4 with <= 100 iterations,
+FP32 CG with FP64 reductions and FP64 residual replacement.  The batch runs on
+concurrent device-resident drivers (nli_inpaint_batch_dev); inputs are resident in
+device memory before the timed region.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -76,59 +77,61 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--images", type=int, default=8, help="1024x1024 images per step")
+    ap.add_argument("--workers", type=int, default=4, help="concurrent per-image drivers")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-compare", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-compare", action="store_true", help="skip the single-image and FP64 runs")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (one process per GPU would "
-              f"inpaint independent images, see the sharding notes)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (images are independent: "
+              f"one process per GPU would take its share of them)", file=sys.stderr)
         sys.exit(2)
 
     nli = importlib.import_module("non-local-inpainting_amd")
     syn = importlib.import_module("non-local-inpainting_amd.synthetic")
-    wl = syn.config_1024()
-    H, W, C = wl.image.shape
+    wls = [syn.config_1024(seed=1 + k) for k in range(args.images)]
+    H, W, C = wls[0].image.shape
     precision = nli.CG_MIXED if args.precision == "mixed" else nli.CG_FP64
     params = workload_params(nli, precision)
     ctx = nli.Context(0)
-    d_in = ctx.to_device(wl.image)
-    d_mask = ctx.to_device(wl.mask)
-    d_out = ctx.dev_alloc(wl.image.nbytes)
+    d_in = [ctx.to_device(w.image) for w in wls]
+    d_mask = [ctx.to_device(w.mask) for w in wls]
+    d_out = [ctx.dev_alloc(w.image.nbytes) for w in wls]
+    Ws, Hs = [W] * len(wls), [H] * len(wls)
 
     for i in range(args.warmup):
-        st = ctx.inpaint_dev(d_in, d_mask, d_out, W, H, C, params)
-        log(f"warmup {i}: {st.seconds_total:.2f} s, iterations per scale {st.iterations}")
+        t0 = time.perf_counter()
+        ctx.inpaint_batch_dev(d_in, d_mask, d_out, Ws, Hs, C, params, args.workers)
+        log(f"warmup {i}: {time.perf_counter() - t0:.2f} s for {len(wls)} images")
 
     ctx.profile(True)
     ctx.lib.nli_synchronize(ctx.h)
     t0 = time.perf_counter()
-    runs = []
     for i in range(args.steps):
-        st = ctx.inpaint_dev(d_in, d_mask, d_out, W, H, C, params)
-        runs.append(st)
-        log(f"step {i}: {st.seconds_total:.2f} s (PatchMatch {st.seconds_patchmatch:.2f} s, update "
-            f"{st.seconds_update:.2f} s), iterations per scale {st.iterations}, CG iterations "
-            f"{st.cg_iterations_total}")
+        t1 = time.perf_counter()
+        stats = ctx.inpaint_batch_dev(d_in, d_mask, d_out, Ws, Hs, C, params, args.workers)
+        log(f"step {i}: {time.perf_counter() - t1:.2f} s for {len(wls)} images; iterations per scale of image 0 "
+            f"{stats[0].iterations}")
     ctx.lib.nli_synchronize(ctx.h)
     elapsed = time.perf_counter() - t0
     kern = {k: ctx.profile_read(k) for k in nli.KERNELS}
     ctx.profile(False)
-    out = ctx.from_device(d_out, wl.image.shape, np.float32)
-    hole = wl.mask.astype(bool)
+    outs = [ctx.from_device(d, w.image.shape, np.float32) for d, w in zip(d_out, wls)]
+    psnrs = [psnr(o[w.mask.astype(bool)], w.truth[w.mask.astype(bool)]) for o, w in zip(outs, wls)]
     ms_per_step = elapsed / args.steps * 1e3
-    value = W * H / 1e6 / (ms_per_step / 1e3)
-    psnr_hole = psnr(out[hole], wl.truth[hole])
+    value = len(wls) * W * H / 1e6 / (ms_per_step / 1e3)
 
-    # roofline of the dominant kernel
+    # roofline of the dominant kernel (event-timed launches of the timed region)
     sweep, cg = kern["pm_sweep"], kern["cg_iteration"]
     if sweep["ms"] >= cg["ms"]:
         achieved = sweep["work"] / sweep["ms"] / 1e9          # flops per ms -> TFLOP/s
         roof = dict(kernel="k_pm_sweep (one PatchMatch sweep)", bound="mfma", achieved=achieved,
                     peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS, traffic=None,
                     launches=sweep["launches"], avg_launch_ms=sweep["ms"] / max(sweep["launches"], 1),
-                    note="FP32 vector work; no MFMA is used (patch distances are not a contraction)")
+                    note="FP32/FP64 vector work (no MFMA: patch distances are not a contraction); algorithmic "
+                         "flops T*(2+R)*P^2*(9C+4) per sweep")
     else:
         achieved = cg["work"] / (cg["ms"] / 1e3) / 1e9
         roof = dict(kernel="k_cg_a + k_cg_b (one CG iteration)", bound="hbm", achieved=achieved,
@@ -136,45 +139,49 @@ def main():
                     launches=cg["launches"], avg_launch_ms=cg["ms"] / max(cg["launches"], 1))
 
     compare = None
-    if not args.no_compare and precision == nli.CG_MIXED:
+    if not args.no_compare:
+        # the same image alone on one stream (device-resident), and in FP64
         t1 = time.perf_counter()
-        st64 = ctx.inpaint_dev(d_in, d_mask, d_out, W, H, C, workload_params(nli, nli.CG_FP64))
-        out64 = ctx.from_device(d_out, wl.image.shape, np.float32)
-        compare = dict(fp64_mpix_s=W * H / 1e6 / (time.perf_counter() - t1),
-                       fp64_psnr_hole_db=psnr(out64[hole], wl.truth[hole]),
-                       fp64_iterations=st64.iterations,
-                       psnr_difference_db=psnr_hole - psnr(out64[hole], wl.truth[hole]))
-        log(f"FP64 comparison run: {compare}")
+        st1 = ctx.inpaint_dev(d_in[0], d_mask[0], d_out[0], W, H, C, params)
+        single = W * H / 1e6 / (time.perf_counter() - t1)
+        out1 = ctx.from_device(d_out[0], wls[0].image.shape, np.float32)
+        t1 = time.perf_counter()
+        ctx.inpaint_dev(d_in[0], d_mask[0], d_out[0], W, H, C, workload_params(nli, nli.CG_FP64))
+        fp64 = W * H / 1e6 / (time.perf_counter() - t1)
+        out64 = ctx.from_device(d_out[0], wls[0].image.shape, np.float32)
+        hole0 = wls[0].mask.astype(bool)
+        compare = dict(single_image_mpix_s=single, single_image_fp64_mpix_s=fp64,
+                       batch_speedup_vs_single=value / single,
+                       psnr_mixed_minus_fp64_db=psnr(out1[hole0], wls[0].truth[hole0]) -
+                       psnr(out64[hole0], wls[0].truth[hole0]),
+                       single_seconds=dict(patchmatch=st1.seconds_patchmatch, update=st1.seconds_update,
+                                           total=st1.seconds_total),
+                       iterations_per_scale=st1.iterations, cg_iterations_total=st1.cg_iterations_total)
+        log(f"single image: {compare}")
 
     cpu = None
     if not args.no_cpu_baseline:
         log(f"CPU baseline sample on {args.cpu_threads} threads ...")
-        cpu = cpu_baseline(wl, runs[-1].iterations, args.cpu_threads)
+        cpu = cpu_baseline(wls[0], stats[0].iterations, args.cpu_threads)
         log(f"CPU baseline: {cpu}")
 
-    last = runs[-1]
     line = {
         "metric": METRIC, "value": value, "unit": "MPix/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": wl.name, "width": W, "height": H, "channels": C,
-                   "hole_fraction": float(wl.mask.mean()), "method": "nlpoisson", "lambda": 0.05, "patch": 9,
-                   "scales": 11, "max_iters": 200, "cg": "tol 1e-4, <= 100 iterations",
-                   "cg_precision": args.precision, "images": 1},
+        "config": {"workload": wls[0].name, "images": len(wls), "width": W, "height": H, "channels": C,
+                   "hole_fraction": float(np.mean([w.mask.mean() for w in wls])), "method": "nlpoisson",
+                   "lambda": 0.05, "patch": 9, "scales": 11, "max_iters": 200, "cg": "tol 1e-4, <= 100 iterations",
+                   "cg_precision": args.precision, "workers": args.workers},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "psnr_hole_db": psnr_hole,
-        "iterations_per_scale": last.iterations,
-        "cg_iterations_total": last.cg_iterations_total,
-        "seconds": {"patchmatch": last.seconds_patchmatch, "update": last.seconds_update,
-                    "total": last.seconds_total},
+        "psnr_hole_db": float(np.mean(psnrs)),
         "kernels": kern,
-        "fp64_compare": compare,
+        "compare": compare,
     }
     print(json.dumps(line), flush=True)
-    ctx.dev_free(d_in)
-    ctx.dev_free(d_mask)
-    ctx.dev_free(d_out)
+    for p in d_in + d_mask + d_out:
+        ctx.dev_free(p)
     ctx.close()
 
 

@@ -241,6 +241,20 @@ int nli_profile_enable(nli_ctx *ctx, int on);
 int nli_profile_read(nli_ctx *ctx, int kernel, int *launches, double *total_ms, double *work,
                      double *units);
 
+/* ---- batches of images ----
+ * Inpaints n independent images (sizes may differ; all have C channels) with the same
+ * parameters.  The images are taken from a shared queue by `workers` concurrent
+ * device-resident drivers, each with its own stream and buffers (0 = one per hardware
+ * queue, 4).  Per-image results are identical to nli_inpaint_dev with the same seed;
+ * stats (optional) receives n records.  _dev: in/mask/out are device pointers. */
+int nli_inpaint_batch(nli_ctx *ctx, const nli_inpaint_params *params, int n, const float *const *in,
+                      const uint8_t *const *mask, const int *W, const int *H, int C, float *const *out,
+                      int workers, nli_inpaint_stats *stats);
+int nli_inpaint_batch_dev(nli_ctx *ctx, const nli_inpaint_params *params, int n,
+                          const float *const *in, const uint8_t *const *mask, const int *W,
+                          const int *H, int C, float *const *out, int workers,
+                          nli_inpaint_stats *stats);
+
 /* fills params with the reference CLI defaults (main.cpp:68-82, 220-233, 254, 266) */
 void nli_inpaint_params_default(nli_inpaint_params *p);
 

@@ -41,7 +41,7 @@ EXPORTS = [
     "nli_upsample", "nli_extended_domain", "nli_scale_masks", "nli_patch_distance",
     "nli_patchmatch", "nli_patchmatch_dev", "nli_update", "nli_update_dev", "nli_poisson_fill",
     "nli_inpaint", "nli_inpaint_dev", "nli_inpaint_params_default", "nli_profile_enable",
-    "nli_profile_read",
+    "nli_profile_read", "nli_inpaint_batch", "nli_inpaint_batch_dev",
 ]
 KERNELS = {"pm_sweep": 0, "pm_init": 1, "cg_iteration": 2, "poisson_coef": 3, "nl_gather": 4}
 
@@ -388,6 +388,38 @@ class Context:
         self._check(self.lib.nli_inpaint_dev(self.h, ct.byref(p), ct.c_void_p(d_in), ct.c_void_p(d_mask), W, H, C,
                                              ct.c_void_p(d_out), ct.byref(st)))
         return stats_of(st)
+
+    # ---- batches ----
+    def inpaint_batch(self, images, masks, params: "Inpainting", workers: int = 0):
+        """Inpaints a list of images (uneven sizes allowed) concurrently; returns
+        (outputs, [RunStats])."""
+        images = [_c(im, np.float32) for im in images]
+        masks = [_c(m, np.uint8) for m in masks]
+        n = len(images)
+        C = images[0].shape[2] if n else 3
+        outs = [np.zeros_like(im) for im in images]
+        PF = ct.POINTER(ct.c_float)
+        PU = ct.POINTER(ct.c_uint8)
+        ins = (PF * n)(*[_ptr(im, _f32p) for im in images])
+        ms = (PU * n)(*[_ptr(m, _u8p) for m in masks])
+        os_ = (PF * n)(*[_ptr(o, _f32p) for o in outs])
+        Ws = (ct.c_int * n)(*[im.shape[1] for im in images])
+        Hs = (ct.c_int * n)(*[im.shape[0] for im in images])
+        st = (InpaintStats * n)()
+        p = params.to_struct()
+        self._check(self.lib.nli_inpaint_batch(self.h, ct.byref(p), n, ins, ms, Ws, Hs, C, os_, workers, st))
+        return outs, [stats_of(st[i]) for i in range(n)]
+
+    def inpaint_batch_dev(self, d_ins, d_masks, d_outs, Ws, Hs, C, params: "Inpainting", workers: int = 0):
+        """Device-resident batch: lists of device pointers (from to_device / dev_alloc)."""
+        n = len(d_ins)
+        V = ct.c_void_p
+        st = (InpaintStats * n)()
+        p = params.to_struct()
+        self._check(self.lib.nli_inpaint_batch_dev(self.h, ct.byref(p), n, (V * n)(*d_ins), (V * n)(*d_masks),
+                                                   (ct.c_int * n)(*Ws), (ct.c_int * n)(*Hs), C, (V * n)(*d_outs),
+                                                   workers, st))
+        return [stats_of(st[i]) for i in range(n)]
 
     # ---- full process ----
     def inpaint(self, image, mask, params: Inpainting):

@@ -632,11 +632,15 @@ Prepared prepare(const DistDesc &d, const float *src, const uint8_t *smask, int 
     a.P = d.patch;
     a.lambda = d.lambda;
     a.lambda_d = d.lambda_d;
-    std::vector<float> wh((size_t)d.patch * d.patch);
-    gaussian_weights(d.patch, d.patch, d.sigma, d.sigma, wh.data());
-    w.weights.ensure(wh.size());
-    NLI_HIP(hipMemcpyAsync(w.weights.p, wh.data(), wh.size() * sizeof(float), hipMemcpyHostToDevice, s));
-    NLI_HIP(hipStreamSynchronize(s));   // wh is pageable and local
+    if (w.w_patch != d.patch || !(w.w_sigma == d.sigma)) {
+        std::vector<float> wh((size_t)d.patch * d.patch);
+        gaussian_weights(d.patch, d.patch, d.sigma, d.sigma, wh.data());
+        w.weights.ensure(wh.size());
+        NLI_HIP(hipMemcpyAsync(w.weights.p, wh.data(), wh.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        NLI_HIP(hipStreamSynchronize(s));   // wh is pageable and local
+        w.w_patch = d.patch;
+        w.w_sigma = d.sigma;
+    }
     a.w = w.weights.p;
     if (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) {
         w.sgrad.ensure((size_t)sW * sH * C * 2);
@@ -650,8 +654,15 @@ Prepared prepare(const DistDesc &d, const float *src, const uint8_t *smask, int 
             a.tgrad = w.tgrad.p;
         }
     }
-    if (need_tiles) {
-        int ntx = (tW + TX - 1) / TX, nty = (tH + TY - 1) / TY;
+    const int ntx = (tW + TX - 1) / TX, nty = (tH + TY - 1) / TY;
+    const bool cached = w.tiles_mask == tmask && w.tiles_W == tW && w.tiles_H == tH && w.tiles_epoch == w.epoch;
+    if (need_tiles && cached) {
+        pr.ntiles = w.ntiles;
+        pr.targets = w.targets;
+        a.tiles = w.tiles.p;
+        a.ntiles_x = ntx;
+        a.ntiles = pr.ntiles;
+    } else if (need_tiles) {
         int nt = ntx * nty;
         w.tflags.ensure(nt);
         w.total.ensure(2);
@@ -662,13 +673,17 @@ Prepared prepare(const DistDesc &d, const float *src, const uint8_t *smask, int 
         w.tiles.ensure(nt);
         w.counts.ensure(nt / 1024 + 2);
         pr.ntiles = compact_indices(w.tflags.p, nt, w.tiles.p, w.counts.p, w.total.p, s);
-        if (prof::enabled()) {
-            NLI_HIP(hipMemcpyAsync(&pr.targets, w.total.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-            NLI_HIP(hipStreamSynchronize(s));
-        }
+        NLI_HIP(hipMemcpyAsync(&pr.targets, w.total.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+        NLI_HIP(hipStreamSynchronize(s));
         a.tiles = w.tiles.p;
         a.ntiles_x = ntx;
         a.ntiles = pr.ntiles;
+        w.tiles_mask = tmask;
+        w.tiles_W = tW;
+        w.tiles_H = tH;
+        w.tiles_epoch = w.epoch;
+        w.ntiles = pr.ntiles;
+        w.targets = pr.targets;
     }
     return pr;
 }

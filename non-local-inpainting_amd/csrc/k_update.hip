@@ -589,11 +589,14 @@ double read_scalar(const double *d, hipStream_t s)
 
 void upload_weights(UpdWork &w, int P, float sigma, hipStream_t s)
 {
+    if (w.w_patch == P && w.w_sigma == sigma) return;
     std::vector<float> wh((size_t)P * P);
     gaussian_weights(P, P, sigma, sigma, wh.data());
     w.weights.ensure(wh.size());
     NLI_HIP(hipMemcpyAsync(w.weights.p, wh.data(), wh.size() * sizeof(float), hipMemcpyHostToDevice, s));
     NLI_HIP(hipStreamSynchronize(s));
+    w.w_patch = P;
+    w.w_sigma = sigma;
 }
 
 // Runs the CG on n unknowns x NC channels; returns with x in a.x[done_at & 1].
@@ -603,10 +606,19 @@ void run_cg(const CGParams &cg, int n, UpdWork &w, CGArgs<T> a, hipStream_t s, C
     const int nb = (int)grid_for(n);
     k_cg_init<NC, T><<<nb, kBlock, 0, s>>>(a, w.x0.p);
     NLI_LAUNCH_CHECK();
+    // Convergence is polled every `poll` iterations without draining the stream: after
+    // each chunk the stop flags are copied to pinned memory behind an event, and the
+    // host waits for the previous chunk's copy while the current chunk runs (kernels
+    // of a converged channel return at once).
     const int poll = 8;
+    if (!w.done_host) {
+        NLI_HIP(hipHostMalloc((void **)&w.done_host, sizeof(int) * 2 * kMaxNC, hipHostMallocDefault));
+        for (hipEvent_t &e : w.poll_ev) NLI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     // algorithmic bytes of one iteration of the reference recurrence (two matvecs):
     // 2 x 13 words per unknown and channel
     const double it_bytes = 2.0 * 13 * sizeof(T) * NC * (double)n;
+    int chunk = 0;
     for (int it = 0; it <= cg.limit + 1; it++) {
         prof::begin(TK_CG, s);
         k_cg_a<NC, T><<<nb, kBlock, 0, s>>>(a, it);
@@ -614,12 +626,17 @@ void run_cg(const CGParams &cg, int n, UpdWork &w, CGArgs<T> a, hipStream_t s, C
         NLI_LAUNCH_CHECK();
         prof::end(TK_CG, s, it_bytes, (double)NC * n);
         if ((it + 1) % poll == 0 && it <= cg.limit) {
-            int done[kMaxNC];
-            NLI_HIP(hipMemcpyAsync(done, a.done_at, sizeof(int) * NC, hipMemcpyDeviceToHost, s));
-            NLI_HIP(hipStreamSynchronize(s));
-            bool all = true;
-            for (int c = 0; c < NC; c++) all = all && done[c] >= 0;
-            if (all) break;
+            int *slot = w.done_host + (chunk & 1) * kMaxNC;
+            NLI_HIP(hipMemcpyAsync(slot, a.done_at, sizeof(int) * NC, hipMemcpyDeviceToHost, s));
+            NLI_HIP(hipEventRecord(w.poll_ev[chunk & 1], s));
+            if (chunk > 0) {
+                const int prev = (chunk - 1) & 1;
+                NLI_HIP(hipEventSynchronize(w.poll_ev[prev]));
+                bool all = true;
+                for (int c = 0; c < NC; c++) all = all && w.done_host[prev * kMaxNC + c] >= 0;
+                if (all) break;
+            }
+            chunk++;
         }
     }
     if (stats) {

@@ -7,11 +7,29 @@
 #include "nli.h"
 #include "nli_internal.h"
 
+#include <atomic>
 #include <cfloat>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
+
+// One batch worker: a stream and the driver's buffers.
+struct nli_worker {
+    hipStream_t stream = nullptr;
+    nli::PMWork pm;
+    nli::UpdWork upd;
+    nli::DriverWork drv;
+    nli::DBuf<float> fin, fout;
+    nli::DBuf<uint8_t> min;
+    ~nli_worker()
+    {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
 
 struct nli_ctx {
     int device = 0;
@@ -24,6 +42,7 @@ struct nli_ctx {
     nli::DBuf<uint8_t> b0, b1, b2, b3;
     nli::DBuf<int32_t> n0, n1;
     nli::DBuf<int> pts, counts, total, nonempty;
+    std::vector<std::unique_ptr<nli_worker>> workers;   // batch workers, created on demand
 };
 
 namespace {
@@ -212,6 +231,81 @@ void report(const nli::InpaintReport &r, nli_inpaint_stats *st)
     st->seconds_patchmatch = r.seconds_patchmatch;
     st->seconds_update = r.seconds_update;
     st->seconds_other = r.seconds_total - r.seconds_patchmatch - r.seconds_update;
+}
+
+}  // namespace
+
+namespace {
+
+// Runs the batch on `workers` host threads, each driving its own stream; images are
+// taken from a shared counter.  staged = false: in/mask/out are device pointers.
+void run_batch(nli_ctx *ctx, const nli_inpaint_params *params, int n, const float *const *in,
+               const uint8_t *const *mask, const int *W, const int *H, int C, float *const *out, int workers,
+               nli_inpaint_stats *stats, bool staged)
+{
+    nli::InpaintConfig cfg = inpaint_config(params);
+    need(n >= 0 && in && mask && W && H && out, "bad batch arguments");
+    need(C >= 1 && C <= 4, "image must have 1 to 4 channels");
+    for (int i = 0; i < n; i++) need(in[i] && mask[i] && out[i] && W[i] > 0 && H[i] > 0, "bad batch image");
+    if (n == 0) return;
+    if (workers <= 0) workers = 4;
+    workers = std::min(workers, n);
+    while ((int)ctx->workers.size() < workers) {
+        auto wk = std::make_unique<nli_worker>();
+        NLI_HIP(hipStreamCreateWithFlags(&wk->stream, hipStreamNonBlocking));
+        ctx->workers.push_back(std::move(wk));
+    }
+    std::atomic<int> next(0);
+    std::mutex mu;
+    std::string first_error;
+    int first_code = 0;
+    auto body = [&](nli_worker *wk) {
+        try {
+            NLI_HIP(hipSetDevice(ctx->device));
+            for (int i = next++; i < n; i = next++) {
+                const size_t np = (size_t)W[i] * H[i];
+                const float *d_in = in[i];
+                const uint8_t *d_m = mask[i];
+                float *d_out = out[i];
+                if (staged) {
+                    d_in = wk->fin.ensure(np * C);
+                    d_m = wk->min.ensure(np);
+                    d_out = wk->fout.ensure(np * C);
+                    NLI_HIP(hipMemcpyAsync((void *)d_in, in[i], np * C * sizeof(float), hipMemcpyHostToDevice,
+                                           wk->stream));
+                    NLI_HIP(hipMemcpyAsync((void *)d_m, mask[i], np, hipMemcpyHostToDevice, wk->stream));
+                }
+                nli::InpaintReport rep{};
+                nli::inpaint(cfg, d_in, d_m, W[i], H[i], C, d_out, rep, wk->drv, wk->pm, wk->upd, wk->stream);
+                if (staged) {
+                    NLI_HIP(hipMemcpyAsync(out[i], d_out, np * C * sizeof(float), hipMemcpyDeviceToHost, wk->stream));
+                    NLI_HIP(hipStreamSynchronize(wk->stream));
+                }
+                if (stats) report(rep, &stats[i]);
+            }
+        } catch (const nli::EmptySource &e) {
+            std::lock_guard<std::mutex> g(mu);
+            if (!first_code) { first_code = NLI_E_EMPTY_SOURCE; first_error = e.what(); }
+        } catch (const nli::HipError &e) {
+            std::lock_guard<std::mutex> g(mu);
+            if (!first_code) { first_code = NLI_E_DEVICE; first_error = e.what(); }
+        } catch (const std::invalid_argument &e) {
+            std::lock_guard<std::mutex> g(mu);
+            if (!first_code) { first_code = NLI_E_ARGUMENT; first_error = e.what(); }
+        } catch (const std::exception &e) {
+            std::lock_guard<std::mutex> g(mu);
+            if (!first_code) { first_code = NLI_E_INTERNAL; first_error = e.what(); }
+        }
+        next = n;   // stop the other workers after a failure or at the end
+    };
+    std::vector<std::thread> threads;
+    for (int k = 1; k < workers; k++) threads.emplace_back(body, ctx->workers[k].get());
+    body(ctx->workers[0].get());
+    for (auto &t : threads) t.join();
+    if (first_code == NLI_E_EMPTY_SOURCE) throw nli::EmptySource();
+    if (first_code == NLI_E_ARGUMENT) throw ArgError(first_error);
+    if (first_code == NLI_E_DEVICE) throw nli::HipError(hipErrorUnknown, first_error);
+    if (first_code) throw std::runtime_error(first_error);
 }
 
 }  // namespace
@@ -492,6 +586,7 @@ int nli_patchmatch_dev(nli_ctx *ctx, const nli_dist_params *dist, const nli_pm_p
         need(C >= 1 && C <= 4, "1 to 4 channels supported");
         need(d.kind != NLI_DIST_L2RGBD || C == 4, "L2 RGBD Distance: Channels do not match.");
         need(init_nnf != out_nnf, "initial field and output must not alias");
+        nli::pm_masks_changed(ctx->pm);
         nli::patchmatch(d, cfg, src, smask, sW, sH, tgt, tmask, tW, tH, C, init_nnf, seed, out_nnf, out_dist, ctx->pm,
                         ctx->stream);
         sync(ctx);
@@ -517,6 +612,7 @@ int nli_patchmatch(nli_ctx *ctx, const nli_dist_params *dist, const nli_pm_param
         const int32_t *d_init = init_nnf ? stage_in(ctx, ctx->n1, init_nnf, tn * 2) : nullptr;
         int32_t *d_out = ctx->n0.ensure(tn * 2);
         float *d_dist = out_dist ? ctx->f2.ensure(tn) : nullptr;
+        nli::pm_masks_changed(ctx->pm);
         nli::patchmatch(d, cfg, d_s, d_sm, sW, sH, d_t, d_tm, tW, tH, C, d_init, seed, d_out, d_dist, ctx->pm,
                         ctx->stream);
         stage_out(ctx, out_nnf, d_out, tn * 2);
@@ -605,6 +701,20 @@ int nli_inpaint_dev(nli_ctx *ctx, const nli_inpaint_params *params, const float 
         nli::inpaint(cfg, in, mask, W, H, C, out, rep, ctx->drv, ctx->pm, ctx->upd, ctx->stream);
         report(rep, stats);
     });
+}
+
+int nli_inpaint_batch(nli_ctx *ctx, const nli_inpaint_params *params, int n, const float *const *in,
+                      const uint8_t *const *mask, const int *W, const int *H, int C, float *const *out, int workers,
+                      nli_inpaint_stats *stats)
+{
+    return guard(ctx, [&] { run_batch(ctx, params, n, in, mask, W, H, C, out, workers, stats, true); });
+}
+
+int nli_inpaint_batch_dev(nli_ctx *ctx, const nli_inpaint_params *params, int n, const float *const *in,
+                          const uint8_t *const *mask, const int *W, const int *H, int C, float *const *out,
+                          int workers, nli_inpaint_stats *stats)
+{
+    return guard(ctx, [&] { run_batch(ctx, params, n, in, mask, W, H, C, out, workers, stats, false); });
 }
 
 int nli_inpaint(nli_ctx *ctx, const nli_inpaint_params *params, const float *in, const uint8_t *mask, int W, int H,

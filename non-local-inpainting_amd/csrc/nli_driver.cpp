@@ -195,6 +195,7 @@ void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask,
             const uint8_t *lhole = dw.mask[L + 1].p;
             extended_domain(lhole, lw, lh, P, dw.ext.p, dw.bscr.p, s);
             scale_masks(dw.ext.p, lw, lh, P / 2, 0, dw.tgt.p, dw.src.p, dw.nonempty.p, s);
+            pm_masks_changed(pmw);
             int32_t *lnnf = next_buf(nullptr, nullptr);
             run_pm(limg, dw.src.p, dw.tgt.p, lw, lh, nullptr, lnnf);
             const int nl = compact_indices(lhole, (int)ln, dw.low_pts.p, dw.counts.p, dw.total.p, s);
@@ -204,6 +205,7 @@ void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask,
             copy_into_hole(dw.up.p, img, dw.hole_pts.p, nh, C, s);
             extended_domain(hole, w, h, P, dw.ext.p, dw.bscr.p, s);
             scale_masks(dw.ext.p, w, h, P / 2, 0, dw.tgt.p, dw.src.p, dw.nonempty.p, s);
+            pm_masks_changed(pmw);
             int32_t *rnnf = next_buf(snnf, nullptr);
             run_pm(dw.up.p, dw.src.p, dw.tgt.p, w, h, snnf, rnnf);
             run_update(img, img, hole, dw.hole_pts.p, nh, dw.ext.p, rnnf, dw.conf.p, w, h);
@@ -213,6 +215,7 @@ void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask,
         // ---- inpaint_internal (:349-417) ----
         extended_domain(hole, w, h, P, dw.ext.p, dw.bscr.p, s);
         scale_masks(dw.ext.p, w, h, P / 2, 1, dw.tgt.p, dw.src.p, dw.nonempty.p, s);
+        pm_masks_changed(pmw);
         int nonempty = 0;
         NLI_HIP(hipMemcpyAsync(&nonempty, dw.nonempty.p, sizeof(int), hipMemcpyDeviceToHost, s));
         NLI_HIP(hipStreamSynchronize(s));

@@ -161,7 +161,18 @@ struct PMWork {          // per-call scratch, grown on demand
     DBuf<int32_t> nnf1;
     DBuf<int> tiles, counts, total;
     DBuf<uint8_t> tflags;
+    // cached patch weights (uploaded for this patch side and sigma)
+    int w_patch = 0;
+    float w_sigma = 0.0f;
+    // cached list of target tiles, valid for (mask pointer, size, epoch)
+    const uint8_t *tiles_mask = nullptr;
+    int tiles_W = 0, tiles_H = 0, ntiles = 0, targets = 0;
+    uint64_t tiles_epoch = ~0ull, epoch = 0;
 };
+
+// The target mask behind a PatchMatch call changed (contents may change behind the
+// same pointer): the cached tile list is rebuilt on the next call.
+inline void pm_masks_changed(PMWork &w) { w.epoch++; }
 
 // Full PatchMatch::calculate on device.  init_nnf may be null (random init).
 // out_nnf receives the field (W x H x 2, (-1,-1) outside the targets); out_dist (optional)
@@ -178,6 +189,19 @@ void patch_distance_field(const DistDesc &d, const float *src, int sW, int sH, c
 
 // ---- image updating (k_update.hip) ----
 struct UpdWork {
+    int w_patch = 0;          // cached patch weights
+    float w_sigma = 0.0f;
+    int *done_host = nullptr; // pinned: pipelined CG convergence polls
+    hipEvent_t poll_ev[2] = {nullptr, nullptr};
+    UpdWork() = default;
+    UpdWork(const UpdWork &) = delete;
+    UpdWork &operator=(const UpdWork &) = delete;
+    ~UpdWork()
+    {
+        if (done_host) (void)hipHostFree(done_host);
+        for (hipEvent_t e : poll_ev)
+            if (e) (void)hipEventDestroy(e);
+    }
     DBuf<float> weights, grad, conf_one;
     DBuf<double> a1, a2, f1, f2, F1, F2, partials, scalars;
     DBuf<double> x0, x1, r, d0, d1, b, u0, stencil;

@@ -11,6 +11,7 @@ namespace {
 
 struct Rec {
     int kernel;
+    hipStream_t stream;
     hipEvent_t a, b;
     double work, units;
 };
@@ -80,7 +81,7 @@ void begin(int kernel, hipStream_t stream)
     State &s = st();
     if (!s.on) return;
     std::lock_guard<std::mutex> g(s.mu);
-    Rec r{kernel, take(s), take(s), 0, 0};
+    Rec r{kernel, stream, take(s), take(s), 0, 0};
     NLI_HIP(hipEventRecord(r.a, stream));
     s.open.push_back(r);
 }
@@ -91,7 +92,7 @@ void end(int kernel, hipStream_t stream, double work, double units)
     if (!s.on) return;
     std::lock_guard<std::mutex> g(s.mu);
     for (size_t i = 0; i < s.open.size(); i++)
-        if (s.open[i].kernel == kernel) {
+        if (s.open[i].kernel == kernel && s.open[i].stream == stream) {
             Rec r = s.open[i];
             s.open.erase(s.open.begin() + i);
             NLI_HIP(hipEventRecord(r.b, stream));
