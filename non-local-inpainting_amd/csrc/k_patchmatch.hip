@@ -127,6 +127,78 @@ struct TgtGlobal {
 // subtractions of gradient.cpp:32-73), so a source patch costs (P+1)^2 C floats of
 // gathers instead of 3 P^2 C.  Arithmetic and summation order are those of
 // patch_distance below, so results are identical.
+// One row (dy) of a gradient distance: cur = image row sy - r + dy, nxt = the row below
+// (the same row again at the bottom image border, so its vertical differences are the
+// exact +0 of gradient.cpp:49-52); column P of both is column x0 + P, or column
+// x0 + P - 1 at the right border (+0 horizontal differences, :34-37).
+template <int KIND, int C, int P, class T>
+__device__ inline void fly_row(const float (&cur)[P + 1][C], const float (&nxt)[P + 1][C],
+                               const float *__restrict__ w, const T &t, int dy, double &dist, double &gdist,
+                               double &dist_d, double &gdist_d)
+{
+#pragma unroll
+    for (int dx = 0; dx < P; dx++) {
+        const float wt = w[dy * P + dx];
+        float g[2 * C];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            g[2 * c] = cur[dx + 1][c] - cur[dx][c];
+            g[2 * c + 1] = nxt[dx][c] - cur[dx][c];
+        }
+        if constexpr (KIND == DIST_L2COMBINED) {
+            float norm = 0.0f, gnorm = 0.0f;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                float d = cur[dx][c] - t.v(c, dx, dy);
+                norm = (c == 0) ? d * d : norm + d * d;
+            }
+#pragma unroll
+            for (int k = 0; k < 2 * C; k++) {
+                float d = g[k] - t.v(C + k, dx, dy);
+                gnorm = (k == 0) ? d * d : gnorm + d * d;
+            }
+            dist += (double)(wt * norm);
+            gdist += (double)(wt * gnorm);
+        } else {   // DIST_L2RGBD, C == 4
+            double norm = 0.0, norm_d = 0.0, gnorm = 0.0, gnorm_d = 0.0;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                float d = cur[dx][c] - t.v(c, dx, dy);
+                float gx = g[2 * c] - t.v(C + 2 * c, dx, dy);
+                float gy = g[2 * c + 1] - t.v(C + 2 * c + 1, dx, dy);
+                if (c == 3) {
+                    norm_d += (double)(d * d);
+                    gnorm_d += (double)(gx * gx + gy * gy);
+                } else {
+                    norm += (double)(d * d);
+                    gnorm += (double)(gx * gx + gy * gy);
+                }
+            }
+            dist += (double)wt * norm;
+            dist_d += (double)wt * norm_d;
+            gdist += (double)wt * gnorm;
+            gdist_d += (double)wt * gnorm_d;
+        }
+    }
+}
+
+template <int C, int P>
+__device__ inline void load_row(float (&row)[P + 1][C], const float *__restrict__ src, int last)
+{
+#pragma unroll
+    for (int dx = 0; dx < P; dx++)
+#pragma unroll
+        for (int c = 0; c < C; c++) row[dx][c] = src[dx * C + c];
+#pragma unroll
+    for (int c = 0; c < C; c++) row[P][c] = src[last * C + c];
+}
+
+// Gradient distances (L2Combined with C = 1 or 3, L2RGBD) for a compile-time patch
+// side: the source gradient is formed on the fly from the image rows the colour term
+// loads anyway (forward differences, the exact float subtractions of
+// gradient.cpp:32-73), so a source patch costs (P+1)^2 C floats of gathers instead of
+// 3 P^2 C.  Arithmetic and summation order are those of patch_distance below, so the
+// results are identical.
 template <int KIND, int C, int PS, class T>
 __device__ float patch_distance_fly(const PMArgs &a, const float *__restrict__ w, int sx, int sy, const T &t,
                                     float bd)
@@ -138,84 +210,31 @@ __device__ float patch_distance_fly(const PMArgs &a, const float *__restrict__ w
     const double lamd = (double)a.lambda_d, lamd1 = (double)(1 - a.lambda_d);
     double dist = 0.0, gdist = 0.0, dist_d = 0.0, gdist_d = 0.0;
     const int x0 = sx - r;
-    const bool has_right = sx + r + 1 < a.sW;
-    float cur[P + 1][C], nxt[P + 1][C];
-    {
-        const float *row = a.src + ((size_t)(sy - r) * a.sW + x0) * C;
-#pragma unroll
-        for (int dx = 0; dx < P; dx++)
-#pragma unroll
-            for (int c = 0; c < C; c++) cur[dx][c] = row[dx * C + c];
-#pragma unroll
-        for (int c = 0; c < C; c++) cur[P][c] = has_right ? row[P * C + c] : 0.0f;
+    const int last = (sx + r + 1 < a.sW) ? P : P - 1;   // right border: repeat the last column
+    const size_t stride = (size_t)a.sW * C;
+    const float *row = a.src + ((size_t)(sy - r) * a.sW + x0) * C;
+    auto partial = [&]() {
+        double v = lam * dist + lam1 * gdist;
+        if constexpr (KIND == DIST_L2RGBD) v = v + lamd * dist_d + lamd1 * gdist_d;
+        return v;
+    };
+    float A[P + 1][C], B[P + 1][C];
+    load_row<C, P>(A, row, last);
+    for (int dy = 0; dy < P; dy += 2) {
+        // rows dy (in A) and dy + 1 (into B)
+        const float *nrow = (sy - r + dy + 1 < a.sH) ? row + stride : row;
+        load_row<C, P>(B, nrow, last);
+        fly_row<KIND, C, P>(A, B, w, t, dy, dist, gdist, dist_d, gdist_d);
+        if (partial() > bound) return INFINITY;
+        if (dy + 1 >= P) break;
+        row = nrow;
+        const float *nrow2 = (sy - r + dy + 2 < a.sH) ? row + stride : row;
+        load_row<C, P>(A, nrow2, last);
+        fly_row<KIND, C, P>(B, A, w, t, dy + 1, dist, gdist, dist_d, gdist_d);
+        if (partial() > bound) return INFINITY;
+        row = nrow2;
     }
-    for (int dy = 0; dy < P; dy++) {
-        const int ys = sy - r + dy;
-        const bool has_down = ys + 1 < a.sH;
-        if (has_down) {
-            const float *row = a.src + ((size_t)(ys + 1) * a.sW + x0) * C;
-#pragma unroll
-            for (int dx = 0; dx < P; dx++)
-#pragma unroll
-                for (int c = 0; c < C; c++) nxt[dx][c] = row[dx * C + c];
-#pragma unroll
-            for (int c = 0; c < C; c++) nxt[P][c] = has_right ? row[P * C + c] : 0.0f;
-        }
-#pragma unroll
-        for (int dx = 0; dx < P; dx++) {
-            const float wt = w[dy * P + dx];
-            float g[2 * C];
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                g[2 * c] = (dx == P - 1 && !has_right) ? 0.0f : cur[dx + 1][c] - cur[dx][c];
-                g[2 * c + 1] = has_down ? nxt[dx][c] - cur[dx][c] : 0.0f;
-            }
-            if constexpr (KIND == DIST_L2COMBINED) {
-                float norm = 0.0f, gnorm = 0.0f;
-#pragma unroll
-                for (int c = 0; c < C; c++) {
-                    float d = cur[dx][c] - t.v(c, dx, dy);
-                    norm = (c == 0) ? d * d : norm + d * d;
-                }
-#pragma unroll
-                for (int k = 0; k < 2 * C; k++) {
-                    float d = g[k] - t.v(C + k, dx, dy);
-                    gnorm = (k == 0) ? d * d : gnorm + d * d;
-                }
-                dist += (double)(wt * norm);
-                gdist += (double)(wt * gnorm);
-            } else {   // DIST_L2RGBD, C == 4
-                double norm = 0.0, norm_d = 0.0, gnorm = 0.0, gnorm_d = 0.0;
-#pragma unroll
-                for (int c = 0; c < C; c++) {
-                    float d = cur[dx][c] - t.v(c, dx, dy);
-                    float gx = g[2 * c] - t.v(C + 2 * c, dx, dy);
-                    float gy = g[2 * c + 1] - t.v(C + 2 * c + 1, dx, dy);
-                    if (c == 3) {
-                        norm_d += (double)(d * d);
-                        gnorm_d += (double)(gx * gx + gy * gy);
-                    } else {
-                        norm += (double)(d * d);
-                        gnorm += (double)(gx * gx + gy * gy);
-                    }
-                }
-                dist += (double)wt * norm;
-                dist_d += (double)wt * norm_d;
-                gdist += (double)wt * gnorm;
-                gdist_d += (double)wt * gnorm_d;
-            }
-        }
-        double partial = lam * dist + lam1 * gdist;
-        if constexpr (KIND == DIST_L2RGBD) partial = partial + lamd * dist_d + lamd1 * gdist_d;
-        if (partial > bound) return INFINITY;
-#pragma unroll
-        for (int dx = 0; dx <= P; dx++)
-#pragma unroll
-            for (int c = 0; c < C; c++) cur[dx][c] = nxt[dx][c];
-    }
-    double total = lam * dist + lam1 * gdist;
-    if constexpr (KIND == DIST_L2RGBD) total = total + lamd * dist_d + lamd1 * gdist_d;
-    return (float)(total / (double)PP);
+    return (float)(partial() / (double)PP);
 }
 
 // Patch distance between the source patch centred at (sx, sy) and the target patch of

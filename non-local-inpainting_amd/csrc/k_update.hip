@@ -18,9 +18,10 @@
 //   |r|^2 < tol |r0|^2 or iteration >= limit).  Unknowns are the hole pixels in
 //   row-major order; A x = a1 x - L x with the reference's anisotropic 5-point
 //   stencil.  Two kernels per iteration for all channels at once; the direction and
-//   iterate updates of the neighbours are recomputed on the fly (double-buffered), and
-//   each reduction finishes in the last block to arrive (fixed summation order, so the
-//   run is deterministic).  Scalars (alpha, beta, stop flags) stay on the device.
+//   iterate updates of the neighbours are recomputed on the fly (double-buffered).
+//   Each kernel stores block partials of its dot product and the next kernel sums
+//   them itself, in a fixed order, so the run is deterministic and needs no grid-wide
+//   fence or atomic.  Scalars (alpha, beta, stop flags) stay on the device.
 // * Mixed precision (precision = 1): the same recurrence with FP32 vectors and FP32
 //   stencil arithmetic, FP64 dot-product accumulation, and the residual recomputed
 //   from an FP64 evaluation of b - A x every `refine_every` iterations.
@@ -42,19 +43,44 @@ constexpr int kMaxP = 15;
 enum { S_NR0 = 0, S_NR, S_RDOTQ, S_ALPHA, S_BETA, S_DONE, S_ITER, S_NPART, S_COUNT };
 
 // ---------------- deterministic block reductions ----------------
+// Butterfly sum over the 64 lanes of a wave: every lane ends with the same bits
+// (each step adds the same two values in both partner lanes).
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block totals of v[] in a fixed order (lanes by butterfly, then the waves in order);
+// every thread of the block receives them.  All threads of the block must call it.
+template <int NC>
+__device__ inline void block_sum(double (&v)[NC])
+{
+    constexpr int kWaves = kBlock / 64;
+    __shared__ double sh[kWaves][NC];
+    const int w = (int)threadIdx.x / 64;
+#pragma unroll
+    for (int c = 0; c < NC; c++) v[c] = wave_sum(v[c]);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < NC; c++) sh[w][c] = v[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        double t = sh[0][c];
+        for (int i = 1; i < kWaves; i++) t += sh[i][c];
+        v[c] = t;
+    }
+    __syncthreads();
+}
+
 template <int NC>
 __device__ inline void block_reduce_store(double (&v)[NC], double *partials, int nb)
 {
-    __shared__ double sh[NC][kBlock];
-    for (int c = 0; c < NC; c++) sh[c][threadIdx.x] = v[c];
-    __syncthreads();
-    for (int off = kBlock / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off)
-            for (int c = 0; c < NC; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + off];
-        __syncthreads();
-    }
+    block_sum<NC>(v);
     if (threadIdx.x == 0)
-        for (int c = 0; c < NC; c++) partials[(size_t)c * nb + blockIdx.x] = sh[c][0];
+        for (int c = 0; c < NC; c++) partials[(size_t)c * nb + blockIdx.x] = v[c];
 }
 
 // After every block stored its partials: the last block to arrive sums them (in block
@@ -386,23 +412,26 @@ struct CGArgs {
     int refine_every;  // T = float: residual recomputed in FP64 every k iterations (0 = never)
 };
 
+// Grid totals of the previous kernel's block partials part[c * nb + block]; every
+// thread of every block forms the same sums in the same order.  The first 4 partials
+// of each thread are loaded unconditionally-in-bounds up front so that the loads
+// overlap the caller's other loads.
 template <int NC>
-__device__ inline void sum_partials(const double *part, int nb, double (&tot)[NC])
+__device__ inline void load_partials(const double *part, int nb, double (&tot)[NC])
 {
-    __shared__ double sh[NC][kBlock];
+#pragma unroll
     for (int c = 0; c < NC; c++) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) s += part[(size_t)c * nb + b];
-        sh[c][threadIdx.x] = s;
+        const double *pc = part + (size_t)c * nb;
+        double v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int b = (int)threadIdx.x + j * kBlock;
+            v[j] = b < nb ? pc[b] : 0.0;
+        }
+        double t = ((v[0] + v[1]) + v[2]) + v[3];
+        for (int b = (int)threadIdx.x + 4 * kBlock; b < nb; b += kBlock) t += pc[b];
+        tot[c] = t;
     }
-    __syncthreads();
-    for (int off = kBlock / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off)
-            for (int c = 0; c < NC; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + off];
-        __syncthreads();
-    }
-    for (int c = 0; c < NC; c++) tot[c] = sh[c][0];
-    __syncthreads();
 }
 
 template <typename T>
@@ -436,58 +465,103 @@ __global__ void k_cg_init(CGArgs<T> a, const double *x0)
     block_reduce_store<NC>(acc, a.prr, a.nb);
 }
 
+// Per-channel scalars of an iteration, formed identically by every thread: the stop
+// test on tot = |r_it|^2 (phase A) and whether the channel still runs.  Block 0 records
+// |r_it|^2 and the stop; a block that reads done_at after block 0 set it in the same
+// kernel sees the channel stopped, which is what its own stop test says.
+template <int NC>
+struct CGScalars {
+    bool run[NC];
+    double coef[NC];   // beta (phase A) or alpha (phase B)
+    bool any;
+};
+
+// Loads of one unknown and its four stencil neighbours from two vectors (u, v);
+// issued before the grid totals are formed so that both memory round trips overlap.
+template <int NC, typename T>
+struct Gather5 {
+    T u[NC][5], v[NC][5];
+    __device__ void load(const T *uv, const T *vv, int n, int k, const int4 &nb, bool with_u)
+    {
+        const int idx[5] = {k, nb.x, nb.y, nb.z, nb.w};
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const size_t o = (size_t)c * n;
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+                const bool ok = idx[j] >= 0;
+                u[c][j] = (with_u && ok) ? uv[o + idx[j]] : (T)0;
+                v[c][j] = ok ? vv[o + idx[j]] : (T)0;
+            }
+        }
+    }
+};
+
 // phase A of iteration it: stop test on |r_it|^2, d_it = r_it + beta d_{it-1} (it > 0),
 // partials of <r_it, A d_it>
 template <int NC, typename T>
-__global__ void k_cg_a(CGArgs<T> a, int it)
+__global__ __launch_bounds__(kBlock) void k_cg_a(CGArgs<T> a, int it)
 {
-    double tot[NC];
-    sum_partials<NC>(a.prr, a.nb, tot);
-    __shared__ int srun[NC];
-    __shared__ double sbeta[NC];
-    if (threadIdx.x == 0)
-        for (int c = 0; c < NC; c++) {
-            bool run = a.done_at[c] < 0;
-            if (run) {
-                double nr = tot[c];
-                double *h = a.hist + (size_t)c * a.hist_len;
-                double nr0 = (it == 0) ? nr : h[0];
-                bool stop = (it == 0) ? !(nr0 > 0) : ((nr < (double)a.tol * nr0) || (it - 1 >= a.limit));
-                if (blockIdx.x == 0) {
-                    h[it] = nr;
-                    if (stop) a.done_at[c] = it;
-                }
-                sbeta[c] = (it == 0 || stop) ? 0.0 : nr / h[it - 1];
-                run = !stop;
-            }
-            srun[c] = run;
-        }
-    __syncthreads();
-    bool any = false;
-    for (int c = 0; c < NC; c++) any = any || srun[c];
-    if (!any) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = k < a.n;
     const T *dp = a.d[it & 1];
     T *dn = a.d[(it + 1) & 1];
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int4 nb = live ? a.nbr[k] : make_int4(-1, -1, -1, -1);
+    Stencil s64{};
+    if (live) s64 = a.st[k];
+    double tot[NC];
+    load_partials<NC>(a.prr, a.nb, tot);
+    int done[NC];
+    double h0[NC], hprev[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        done[c] = a.done_at[c];
+        const double *h = a.hist + (size_t)c * a.hist_len;
+        h0[c] = it > 0 ? h[0] : 0.0;
+        hprev[c] = it > 0 ? h[it - 1] : 0.0;
+    }
+    Gather5<NC, T> g;   // u = r, v = d_{it-1}
+    if (live) g.load(a.r, dp, a.n, k, nb, it > 0);
+    block_sum<NC>(tot);
+
+    bool any = false;
+    T beta[NC];
+    bool run[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        run[c] = done[c] < 0;
+        beta[c] = (T)0;
+        if (run[c]) {
+            const double nr = tot[c];
+            const double nr0 = (it == 0) ? nr : h0[c];
+            const bool stop = (it == 0) ? !(nr0 > 0) : ((nr < (double)a.tol * nr0) || (it - 1 >= a.limit));
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                a.hist[(size_t)c * a.hist_len + it] = nr;
+                if (stop) a.done_at[c] = it;
+            }
+            beta[c] = (it == 0 || stop) ? (T)0 : (T)(nr / hprev[c]);
+            run[c] = !stop;
+        }
+        any = any || run[c];
+    }
+    if (!any) return;   // uniform over the grid
     double acc[NC];
+#pragma unroll
     for (int c = 0; c < NC; c++) acc[c] = 0.0;
-    if (k < a.n) {
-        StencilOf<T> s(a.st[k]);
-        int4 nb = a.nbr[k];
+    if (live) {
+        const StencilOf<T> s(s64);
+        const bool has[5] = {true, nb.x >= 0, nb.y >= 0, nb.z >= 0, nb.w >= 0};
+#pragma unroll
         for (int c = 0; c < NC; c++) {
-            if (!srun[c]) continue;
-            size_t o = (size_t)c * a.n;
-            const T *rv = a.r + o, *dv = dp + o;
-            T beta = (T)sbeta[c];
-            auto D = [&](int j) { return it == 0 ? dv[j] : rv[j] + beta * dv[j]; };
-            T dc = D(k);
-            T ut = nb.x >= 0 ? D(nb.x) : (T)0;
-            T ul = nb.y >= 0 ? D(nb.y) : (T)0;
-            T ub = nb.z >= 0 ? D(nb.z) : (T)0;
-            T ur = nb.w >= 0 ? D(nb.w) : (T)0;
-            T L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - dc * s.sum;
-            dn[o + k] = dc;
-            acc[c] = (double)rv[k] * (double)(s.a1 * dc - L);
+            if (!run[c]) continue;
+            T D[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++)
+                D[j] = !has[j] ? (T)0 : (it == 0 ? g.v[c][j] : g.u[c][j] + beta[c] * g.v[c][j]);
+            const T L = D[1] * s.at + D[2] * s.al + D[3] * s.ab + D[4] * s.ar - D[0] * s.sum;
+            dn[(size_t)c * a.n + k] = D[0];
+            const T rk = it == 0 ? g.v[c][0] : g.u[c][0];   // r_0 = d_0 (same bits)
+            acc[c] = (double)rk * (double)(s.a1 * D[0] - L);
         }
     }
     block_reduce_store<NC>(acc, a.prq, a.nb);
@@ -496,55 +570,64 @@ __global__ void k_cg_a(CGArgs<T> a, int it)
 // phase B of iteration it: alpha = |r_it|^2 / <r_it, A d_it>, x_{it+1} = x_it + alpha d_it,
 // r_{it+1} = b - A x_{it+1}, partials of |r_{it+1}|^2
 template <int NC, typename T>
-__global__ void k_cg_b(CGArgs<T> a, int it)
+__global__ __launch_bounds__(kBlock) void k_cg_b(CGArgs<T> a, int it)
 {
-    double tot[NC];
-    sum_partials<NC>(a.prq, a.nb, tot);
-    __shared__ int srun[NC];
-    __shared__ double salpha[NC];
-    if (threadIdx.x == 0)
-        for (int c = 0; c < NC; c++) {
-            bool run = a.done_at[c] < 0;
-            srun[c] = run;
-            salpha[c] = run ? a.hist[(size_t)c * a.hist_len + it] / tot[c] : 0.0;
-        }
-    __syncthreads();
-    bool any = false;
-    for (int c = 0; c < NC; c++) any = any || srun[c];
-    if (!any) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = k < a.n;
     const T *xp = a.x[it & 1];
     T *xn = a.x[(it + 1) & 1];
     const T *dcur = a.d[(it + 1) & 1];
+    const int4 nb = live ? a.nbr[k] : make_int4(-1, -1, -1, -1);
+    Stencil s64{};
+    if (live) s64 = a.st[k];
+    double tot[NC];
+    load_partials<NC>(a.prq, a.nb, tot);
+    int done[NC];
+    double hit[NC], bk[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        done[c] = a.done_at[c];
+        hit[c] = a.hist[(size_t)c * a.hist_len + it];
+        bk[c] = live ? a.b[(size_t)c * a.n + k] : 0.0;
+    }
+    Gather5<NC, T> g;   // u = x_it, v = d_it
+    if (live) g.load(xp, dcur, a.n, k, nb, true);
+    block_sum<NC>(tot);
+
+    bool any = false;
+    bool run[NC];
+    T alpha[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        run[c] = done[c] < 0;
+        alpha[c] = run[c] ? (T)(hit[c] / tot[c]) : (T)0;
+        any = any || run[c];
+    }
+    if (!any) return;   // uniform over the grid
     const bool refine = std::is_same<T, float>::value && a.refine_every > 0 && ((it + 1) % a.refine_every == 0);
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[NC];
+#pragma unroll
     for (int c = 0; c < NC; c++) acc[c] = 0.0;
-    if (k < a.n) {
-        Stencil s64 = a.st[k];
-        StencilOf<T> s(s64);
-        int4 nb = a.nbr[k];
+    if (live) {
+        const StencilOf<T> s(s64);
+        const bool has[5] = {true, nb.x >= 0, nb.y >= 0, nb.z >= 0, nb.w >= 0};
+#pragma unroll
         for (int c = 0; c < NC; c++) {
-            if (!srun[c]) continue;
-            size_t o = (size_t)c * a.n;
-            const T *xv = xp + o, *dv = dcur + o;
-            T alpha = (T)salpha[c];
-            auto X = [&](int j) { return xv[j] + alpha * dv[j]; };
-            T xc = X(k);
-            T ut = nb.x >= 0 ? X(nb.x) : (T)0;
-            T ul = nb.y >= 0 ? X(nb.y) : (T)0;
-            T ub = nb.z >= 0 ? X(nb.z) : (T)0;
-            T ur = nb.w >= 0 ? X(nb.w) : (T)0;
+            if (!run[c]) continue;
+            T X[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++) X[j] = has[j] ? g.u[c][j] + alpha[c] * g.v[c][j] : (T)0;
             double rv;
             if (refine) {   // residual replacement: b - A x evaluated in FP64
-                double L = (double)ut * s64.at + (double)ul * s64.al + (double)ub * s64.ab + (double)ur * s64.ar -
-                           (double)xc * s64.sum;
-                rv = a.b[o + k] - s64.a1 * (double)xc + L;
+                const double L = (double)X[1] * s64.at + (double)X[2] * s64.al + (double)X[3] * s64.ab +
+                                 (double)X[4] * s64.ar - (double)X[0] * s64.sum;
+                rv = bk[c] - s64.a1 * (double)X[0] + L;
             } else {
-                T L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - xc * s.sum;
-                rv = (double)((T)a.b[o + k] - s.a1 * xc + L);
+                const T L = X[1] * s.at + X[2] * s.al + X[3] * s.ab + X[4] * s.ar - X[0] * s.sum;
+                rv = (double)((T)bk[c] - s.a1 * X[0] + L);
             }
-            xn[o + k] = xc;
-            a.r[o + k] = (T)rv;
+            xn[(size_t)c * a.n + k] = X[0];
+            a.r[(size_t)c * a.n + k] = (T)rv;
             acc[c] = rv * rv;
         }
     }
