@@ -29,6 +29,21 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_pm_sweep_rgb1024_nlpoisson_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per sweep from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (copy-calibrated), or None if it is absent."""
+    try:
+        for line in open(TRAFFIC_SUMMARY):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -82,6 +97,10 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the single-image and FP64 runs")
+    ap.add_argument("--no-breakdown", action="store_true", help="skip the all-kernel timing batch")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run 3 device-to-device copies of MIB MiB (the known-size copy that "
+                         "calibrates hardware byte counters)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -101,12 +120,22 @@ def main():
     d_out = [ctx.dev_alloc(w.image.nbytes) for w in wls]
     Ws, Hs = [W] * len(wls), [H] * len(wls)
 
+    if args.copy_probe > 0:
+        nbytes = args.copy_probe << 20
+        a, b = ctx.dev_alloc(nbytes), ctx.dev_alloc(nbytes)
+        for _ in range(3):
+            ctx.copy_d2d(b, a, nbytes)
+        ctx.dev_free(a)
+        ctx.dev_free(b)
+        log(f"copy probe: 3 device copies of {nbytes} bytes")
+
     for i in range(args.warmup):
         t0 = time.perf_counter()
         ctx.inpaint_batch_dev(d_in, d_mask, d_out, Ws, Hs, C, params, args.workers)
         log(f"warmup {i}: {time.perf_counter() - t0:.2f} s for {len(wls)} images")
 
-    ctx.profile(True)
+    # timed steps: only the dominant kernel (the PatchMatch sweep) carries timing events
+    ctx.profile(["pm_sweep"])
     ctx.lib.nli_synchronize(ctx.h)
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -116,27 +145,30 @@ def main():
             f"{stats[0].iterations}")
     ctx.lib.nli_synchronize(ctx.h)
     elapsed = time.perf_counter() - t0
-    kern = {k: ctx.profile_read(k) for k in nli.KERNELS}
+    sweep = ctx.profile_read("pm_sweep")
     ctx.profile(False)
     outs = [ctx.from_device(d, w.image.shape, np.float32) for d, w in zip(d_out, wls)]
     psnrs = [psnr(o[w.mask.astype(bool)], w.truth[w.mask.astype(bool)]) for o, w in zip(outs, wls)]
     ms_per_step = elapsed / args.steps * 1e3
     value = len(wls) * W * H / 1e6 / (ms_per_step / 1e3)
 
-    # roofline of the dominant kernel (event-timed launches of the timed region)
-    sweep, cg = kern["pm_sweep"], kern["cg_iteration"]
-    if sweep["ms"] >= cg["ms"]:
-        achieved = sweep["work"] / sweep["ms"] / 1e9          # flops per ms -> TFLOP/s
-        roof = dict(kernel="k_pm_sweep (one PatchMatch sweep)", bound="mfma", achieved=achieved,
-                    peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS, traffic=None,
-                    launches=sweep["launches"], avg_launch_ms=sweep["ms"] / max(sweep["launches"], 1),
-                    note="FP32/FP64 vector work (no MFMA: patch distances are not a contraction); algorithmic "
-                         "flops T*(2+R)*P^2*(9C+4) per sweep")
-    else:
-        achieved = cg["work"] / (cg["ms"] / 1e3) / 1e9
-        roof = dict(kernel="k_cg_a + k_cg_b (one CG iteration)", bound="hbm", achieved=achieved,
-                    peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS, traffic=None,
-                    launches=cg["launches"], avg_launch_ms=cg["ms"] / max(cg["launches"], 1))
+    # roofline of the dominant kernel from the event-timed sweeps of the timed region
+    achieved = sweep["work"] / sweep["ms"] / 1e9          # flops per ms -> TFLOP/s
+    roof = dict(kernel="k_pm_sweep (one PatchMatch sweep)", bound="mfma", achieved=achieved,
+                peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS, traffic=measured_traffic(),
+                traffic_source="profiles/k_pm_sweep_rgb1024_nlpoisson_traffic.md (bytes per sweep)",
+                launches=sweep["launches"], avg_launch_ms=sweep["ms"] / max(sweep["launches"], 1),
+                note="FP32/FP64 vector work (no MFMA: patch distances are not a contraction); algorithmic "
+                     "flops T*(2+R)*P^2*(9C+4) per sweep")
+
+    # per-kernel breakdown: one more batch with every kernel timed (not part of the timed steps;
+    # concurrent streams make each kernel's event time include the others' work)
+    kern = None
+    if not args.no_breakdown:
+        ctx.profile(True)
+        ctx.inpaint_batch_dev(d_in, d_mask, d_out, Ws, Hs, C, params, args.workers)
+        kern = {k: ctx.profile_read(k) for k in nli.KERNELS}
+        ctx.profile(False)
 
     compare = None
     if not args.no_compare:
@@ -149,8 +181,15 @@ def main():
         ctx.inpaint_dev(d_in[0], d_mask[0], d_out[0], W, H, C, workload_params(nli, nli.CG_FP64))
         fp64 = W * H / 1e6 / (time.perf_counter() - t1)
         out64 = ctx.from_device(d_out[0], wls[0].image.shape, np.float32)
+        # the plain drop-in: the C++ plugin classes in the reference's composition, one
+        # C-ABI call per reference call on host buffers
+        t1 = time.perf_counter()
+        out_plain, _ = ctx.inpaint_plain(wls[0].image, wls[0].mask, params)
+        plain = W * H / 1e6 / (time.perf_counter() - t1)
         hole0 = wls[0].mask.astype(bool)
-        compare = dict(single_image_mpix_s=single, single_image_fp64_mpix_s=fp64,
+        compare = dict(plain_dropin_mpix_s=plain, batch_speedup_vs_plain=value / plain,
+                       plain_equals_device_driver=bool(np.array_equal(out_plain, out1)),
+                       single_image_mpix_s=single, single_image_fp64_mpix_s=fp64,
                        batch_speedup_vs_single=value / single,
                        psnr_mixed_minus_fp64_db=psnr(out1[hole0], wls[0].truth[hole0]) -
                        psnr(out64[hole0], wls[0].truth[hole0]),

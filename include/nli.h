@@ -141,6 +141,8 @@ int nli_dev_alloc(nli_ctx *ctx, uint64_t bytes, void **out);
 int nli_dev_free(nli_ctx *ctx, void *p);
 int nli_copy_h2d(nli_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 int nli_copy_d2h(nli_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+/* device-to-device copy on the context's stream (returns when it has completed) */
+int nli_copy_d2d(nli_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
 /* ---- numerics ---- */
 /* replaces Gradient::calculate (gradient.h:25, gradient.cpp:20-76): out W*H*2C */
@@ -178,6 +180,12 @@ int nli_extended_domain(nli_ctx *ctx, const uint8_t *hole, int W, int H, int pat
  * Returns NLI_E_EMPTY_SOURCE when erode = 1 and no source pixel survives the erosion. */
 int nli_scale_masks(nli_ctx *ctx, const uint8_t *hole, int W, int H, int patch, int erode,
                     uint8_t *ext, uint8_t *target, uint8_t *source);
+
+/* the NNF scaling of ImageInpainting::propagate_weights (image_inpainting.cpp:515-529):
+ * out (W*H*2) = (-1,-1) except at round(p * scale) for the lower-level hole points p with
+ * an entry, which receive round(nnf(p) * scale); scale = (float)W / lW, (float)H / lH. */
+int nli_scale_nnf(nli_ctx *ctx, const int32_t *lower_nnf, const uint8_t *lower_hole, int lW, int lH,
+                  int W, int H, int32_t *out);
 
 /* ---- patch distances and PatchMatch ---- */
 /* APatchDistance::calculate (a_patch_distance.h:32-35) for every target point of a fixed
@@ -231,13 +239,17 @@ int nli_inpaint_dev(nli_ctx *ctx, const nli_inpaint_params *params, const float 
  * With timing on, HIP events recorded on the context's stream bracket the launches of
  * the kernels below; nli_profile_read synchronises and returns, for one kernel, the
  * launches, their total duration, their algorithmic work (flops for PatchMatch sweeps,
- * bytes for CG iterations) and the elements processed.  Enabling resets the totals. */
+ * bytes for CG iterations) and the elements processed.  `kernels` is a bitmask of
+ * (1 << NLI_KERNEL_*) selecting the kernels to time (NLI_PROFILE_ALL: all of them,
+ * 0: timing off); every call resets the totals.  Each timed launch adds an event pair
+ * to its stream, so time only the kernels of interest in throughput runs. */
 #define NLI_KERNEL_PM_SWEEP 0
 #define NLI_KERNEL_PM_INIT 1
 #define NLI_KERNEL_CG_ITERATION 2
 #define NLI_KERNEL_POISSON_COEF 3
 #define NLI_KERNEL_NL_GATHER 4
-int nli_profile_enable(nli_ctx *ctx, int on);
+#define NLI_PROFILE_ALL (-1)
+int nli_profile_enable(nli_ctx *ctx, int kernels);
 int nli_profile_read(nli_ctx *ctx, int kernel, int *launches, double *total_ms, double *work,
                      double *units);
 

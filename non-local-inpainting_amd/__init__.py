@@ -20,6 +20,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnli.so")
+PLUGINS_PATH = os.path.join(HERE, "libnli_plugins.so")
 
 NLI_OK = 0
 NLI_E_ARGUMENT = -1
@@ -35,14 +36,16 @@ CG_FP64, CG_MIXED = 0, 1
 # every symbol include/nli.h declares
 EXPORTS = [
     "nli_ctx_create", "nli_ctx_destroy", "nli_last_error", "nli_synchronize", "nli_version",
-    "nli_dev_alloc", "nli_dev_free", "nli_copy_h2d", "nli_copy_d2h",
+    "nli_dev_alloc", "nli_dev_free", "nli_copy_h2d", "nli_copy_d2h", "nli_copy_d2d",
     "nli_gradient", "nli_distance_transform", "nli_confidence", "nli_sample_size",
     "nli_subsampling_rate", "nli_downsample_with_mask", "nli_downsample", "nli_downsample_mask",
-    "nli_upsample", "nli_extended_domain", "nli_scale_masks", "nli_patch_distance",
+    "nli_upsample", "nli_extended_domain", "nli_scale_masks", "nli_scale_nnf", "nli_patch_distance",
     "nli_patchmatch", "nli_patchmatch_dev", "nli_update", "nli_update_dev", "nli_poisson_fill",
     "nli_inpaint", "nli_inpaint_dev", "nli_inpaint_params_default", "nli_profile_enable",
     "nli_profile_read", "nli_inpaint_batch", "nli_inpaint_batch_dev",
 ]
+# symbols of libnli_plugins.so (include/nli_plugins.hpp)
+PLUGIN_EXPORTS = ["nli_plain_inpaint", "nli_plain_last_error"]
 KERNELS = {"pm_sweep": 0, "pm_init": 1, "cg_iteration": 2, "poisson_coef": 3, "nl_gather": 4}
 
 
@@ -118,6 +121,24 @@ def load_library(path: str = LIB_PATH) -> ct.CDLL:
     lib.nli_ctx_destroy.argtypes = [ct.c_void_p]
     lib.nli_last_error.argtypes = [ct.c_void_p]
     _lib = lib
+    return lib
+
+
+_plugins = None
+
+
+def load_plugins() -> ct.CDLL:
+    """Load libnli_plugins.so, the C++ plugin classes composed as the reference's main()
+    (raises if it has not been built)."""
+    global _plugins
+    if _plugins is not None:
+        return _plugins
+    load_library()
+    if not os.path.exists(PLUGINS_PATH):
+        raise ImportError(f"{PLUGINS_PATH} is not built: run __graft_entry__.build()")
+    lib = ct.CDLL(PLUGINS_PATH)
+    lib.nli_plain_last_error.restype = ct.c_char_p
+    _plugins = lib
     return lib
 
 
@@ -349,8 +370,16 @@ class Context:
         return (img, total.value, st) if stats else (img, total.value)
 
     # ---- kernel timing ----
-    def profile(self, on: bool = True):
-        self._check(self.lib.nli_profile_enable(self.h, int(on)))
+    def profile(self, on=True):
+        """Time kernels with HIP events: True = all, False = none, or an iterable of
+        kernel names (keys of KERNELS).  Resets the totals."""
+        if on is True or on is False:
+            mask = -1 if on else 0
+        else:
+            mask = 0
+            for k in on:
+                mask |= 1 << KERNELS[k]
+        self._check(self.lib.nli_profile_enable(self.h, mask))
 
     def profile_read(self, kernel: str) -> dict:
         n = ct.c_int()
@@ -380,6 +409,9 @@ class Context:
         self._check(self.lib.nli_copy_d2h(self.h, out.ctypes.data_as(ct.c_void_p), ct.c_void_p(p),
                                           ct.c_uint64(out.nbytes)))
         return out
+
+    def copy_d2d(self, dst: int, src: int, nbytes: int):
+        self._check(self.lib.nli_copy_d2d(self.h, ct.c_void_p(dst), ct.c_void_p(src), ct.c_uint64(nbytes)))
 
     def inpaint_dev(self, d_in: int, d_mask: int, d_out: int, W: int, H: int, C: int, params: "Inpainting"):
         """The device-resident process(): inputs and output already in device memory."""
@@ -431,6 +463,27 @@ class Context:
         st = InpaintStats()
         self._check(self.lib.nli_inpaint(self.h, ct.byref(p), _ptr(image, _f32p), _ptr(mask, _u8p), W, H, C,
                                          _ptr(out, _f32p), ct.byref(st)))
+        return out, stats_of(st)
+
+
+    def inpaint_plain(self, image, mask, params: Inpainting):
+        """The plain drop-in: the reference's main() composition of the C++ plugin classes
+        (HipPatchMatch, HipPatchNonLocal*, ImageInpainting), one C-ABI call per reference
+        call on host buffers."""
+        lib = load_plugins()
+        image = _c(image, np.float32)
+        mask = _c(mask, np.uint8)
+        H, W, C = image.shape
+        out = np.zeros_like(image)
+        p = params.to_struct()
+        st = InpaintStats()
+        rc = lib.nli_plain_inpaint(self.h, ct.byref(p), _ptr(image, _f32p), _ptr(mask, _u8p), W, H, C,
+                                   _ptr(out, _f32p), ct.byref(st))
+        if rc != NLI_OK:
+            msg = lib.nli_plain_last_error().decode(errors="replace")
+            if rc == NLI_E_EMPTY_SOURCE:
+                raise EmptySourceError(rc, msg)
+            raise NliError(rc, msg)
         return out, stats_of(st)
 
 

@@ -379,11 +379,19 @@ int nli_copy_d2h(nli_ctx *ctx, void *dst, const void *src, uint64_t bytes)
     });
 }
 
-int nli_profile_enable(nli_ctx *ctx, int on)
+int nli_copy_d2d(nli_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+    return guard(ctx, [&] {
+        NLI_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        sync(ctx);
+    });
+}
+
+int nli_profile_enable(nli_ctx *ctx, int kernels)
 {
     return guard(ctx, [&] {
         sync(ctx);
-        nli::prof::enable(on != 0);
+        nli::prof::enable((unsigned)kernels);
     });
 }
 
@@ -521,6 +529,25 @@ int nli_extended_domain(nli_ctx *ctx, const uint8_t *hole, int W, int H, int pat
         uint8_t *scr = ctx->b2.ensure(n);
         nli::extended_domain(d_h, W, H, patch, d_e, scr, ctx->stream);
         stage_out(ctx, out, d_e, n);
+        sync(ctx);
+    });
+}
+
+int nli_scale_nnf(nli_ctx *ctx, const int32_t *lower_nnf, const uint8_t *lower_hole, int lW, int lH, int W, int H,
+                  int32_t *out)
+{
+    return guard(ctx, [&] {
+        need(lower_nnf && lower_hole && out && lW > 0 && lH > 0 && W > 0 && H > 0, "bad NNF scaling arguments");
+        const size_t ln = (size_t)lW * lH, n = (size_t)W * H;
+        const int32_t *d_l = stage_in(ctx, ctx->n0, lower_nnf, 2 * ln);
+        const uint8_t *d_h = stage_in(ctx, ctx->b0, lower_hole, ln);
+        int *pts = ctx->pts.ensure(ln);
+        int *counts = ctx->counts.ensure(ln / 1024 + 2);
+        int *total = ctx->total.ensure(2);
+        int32_t *d_o = ctx->n1.ensure(2 * n);
+        const int nl = nli::compact_indices(d_h, (int)ln, pts, counts, total, ctx->stream);
+        nli::scale_nnf(d_l, pts, nl, lW, lH, W, H, d_o, ctx->stream);
+        stage_out(ctx, out, d_o, 2 * n);
         sync(ctx);
     });
 }

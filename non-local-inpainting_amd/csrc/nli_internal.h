@@ -93,8 +93,8 @@ enum TimedKernel {
     TK_COUNT = 6
 };
 namespace prof {
-bool enabled();
-void enable(bool on);
+// kernels: bitmask over TimedKernel ids (0 = timing off)
+void enable(unsigned kernels);
 // brackets launches on stream s; `work` is the algorithmic flops or bytes of the
 // bracketed launches, `units` the elements they process
 void begin(int kernel, hipStream_t s);

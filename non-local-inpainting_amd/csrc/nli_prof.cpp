@@ -18,7 +18,7 @@ struct Rec {
 
 struct State {
     std::mutex mu;
-    bool on = false;
+    unsigned mask = 0;   // bit k: kernel k is timed
     std::vector<Rec> open;           // begun, not ended (per kernel at most one)
     std::vector<Rec> done;
     std::vector<hipEvent_t> pool;
@@ -62,14 +62,12 @@ void drain(State &s)
 
 }  // namespace
 
-bool enabled() { return st().on; }
-
-void enable(bool on)
+void enable(unsigned kernels)
 {
     State &s = st();
     std::lock_guard<std::mutex> g(s.mu);
     drain(s);
-    s.on = on;
+    s.mask = kernels & ((1u << TK_COUNT) - 1);
     for (int k = 0; k < TK_COUNT; k++) {
         s.launches[k] = 0;
         s.ms[k] = s.work[k] = s.units[k] = 0;
@@ -79,7 +77,7 @@ void enable(bool on)
 void begin(int kernel, hipStream_t stream)
 {
     State &s = st();
-    if (!s.on) return;
+    if (!((s.mask >> kernel) & 1u)) return;
     std::lock_guard<std::mutex> g(s.mu);
     Rec r{kernel, stream, take(s), take(s), 0, 0};
     NLI_HIP(hipEventRecord(r.a, stream));
@@ -89,7 +87,7 @@ void begin(int kernel, hipStream_t stream)
 void end(int kernel, hipStream_t stream, double work, double units)
 {
     State &s = st();
-    if (!s.on) return;
+    if (!((s.mask >> kernel) & 1u)) return;
     std::lock_guard<std::mutex> g(s.mu);
     for (size_t i = 0; i < s.open.size(); i++)
         if (s.open[i].kernel == kernel && s.open[i].stream == stream) {

@@ -10,8 +10,8 @@ import pytest
 from conftest import ROOT, bitexact
 
 
-def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "nli.h")).read()
+def declared_symbols(header="nli.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(nli_[a-z0-9_]+)\s*\(", text)))
 
@@ -23,6 +23,26 @@ def test_library_exports_every_declared_symbol(nli):
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert sorted(nli.EXPORTS) == names
+
+
+def test_plugin_library_exports_its_declared_symbols(nli):
+    lib = nli.load_plugins()
+    names = declared_symbols("nli_plugins.hpp")
+    own = [n for n in names if n.startswith("nli_plain")]
+    assert own == sorted(nli.PLUGIN_EXPORTS)
+    missing = [n for n in own if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_plain_inpaint_rejects_null_context(nli):
+    lib = nli.load_plugins()
+    p = nli.Inpainting().to_struct()
+    img = np.zeros((8, 8, 3), np.float32)
+    mask = np.zeros((8, 8), np.uint8)
+    out = np.zeros_like(img)
+    rc = lib.nli_plain_inpaint(None, ct.byref(p), img.ctypes.data_as(ct.c_void_p), mask.ctypes.data_as(ct.c_void_p),
+                               8, 8, 3, out.ctypes.data_as(ct.c_void_p), None)
+    assert rc == nli.NLI_E_ARGUMENT
 
 
 def test_version(nli):
