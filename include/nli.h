@@ -220,6 +220,45 @@ int nli_update_dev(nli_ctx *ctx, const nli_upd_params *upd, float *image, const 
                    const float *conf, int W, int H, int C, double *total_diff,
                    nli_cg_stats *stats);
 
+/* ---- the NL-Poisson patch energy, its gradient and the adjoint aggregation ----
+ * For a fixed NNF and confidence, the NL-Poisson update (patch_non_local_poisson.cpp:
+ * 36-232) minimises over the hole pixels the patch energy
+ *   E(u) = sum_{p in ext} sum_{h: p+h, nnf(p)+h in the image} (double)(conf(p) g(h)) *
+ *          sum_c [ lambda (u(p+h) - u(s))^2 + (1-lambda) |grad u(p+h) - grad u(s)|^2 ],
+ *   s = nnf(p) + h, grad = forward differences (0 on the last column / row), lambda and
+ *   1-lambda in float as the reference's coefficients take them,
+ * whose terms are those the reference accumulates into a1, a2, f1, F.  Its gradient is
+ * twice the residual of the reference's linear system on the hole pixels.  No
+ * counterpart exists in the reference: these serve differentiation (energy gradient
+ * with respect to the pixels) and adjoint checks. */
+typedef struct {
+    int patch;       /* odd, <= 15 */
+    float sigma;     /* Gaussian sigma of the intra-patch weights */
+    float lambda;
+} nli_energy_params;
+/* E(u); with px >= 0 only the terms that read pixel (px, py) channel pc, evaluated with
+ * that value increased by delta (FP64; for finite differences).  conf may be NULL. */
+int nli_patch_energy(nli_ctx *ctx, const nli_energy_params *params, const float *image,
+                     const uint8_t *ext, const int32_t *nnf, const float *conf, int W, int H, int C,
+                     int px, int py, int pc, double delta, double *energy);
+/* dE/du for every pixel and channel (W*H*C doubles, HWC): target-side terms by gather,
+ * source-side terms by scatter, NNF re-read instead of stored per-patch weights */
+int nli_energy_gradient(nli_ctx *ctx, const nli_energy_params *params, const float *image,
+                        const uint8_t *ext, const int32_t *nnf, const float *conf, int W, int H,
+                        int C, double *grad);
+/* patch extraction: for the extended-domain centres p (row-major, *n_centres of them),
+ * patches[((e*P*P) + h)*C + c] = u(p+h) (side 0) or u(nnf(p)+h) (side 1), 0 where p+h
+ * or nnf(p)+h leaves the image; capacity = doubles available in patches */
+int nli_patch_extract(nli_ctx *ctx, int patch, const double *u, const uint8_t *ext,
+                      const int32_t *nnf, int W, int H, int C, int side, double *patches,
+                      uint64_t capacity, int *n_centres);
+/* the adjoint of nli_patch_extract (the aggregation of patches back onto pixels):
+ * side 0 gathers at each pixel the entries of the centres p = q - h, side 1 scatters to
+ * nnf(p) + h; out W*H*C doubles */
+int nli_adjoint_aggregate(nli_ctx *ctx, int patch, const double *patches, int n_centres,
+                          const uint8_t *ext, const int32_t *nnf, int W, int H, int C, int side,
+                          double *out);
+
 /* ---- coarsest-level initialisation ---- */
 /* replaces poisson_solver_separable_simplest (3rdparty/simpois/simpois.c:281-291) as
  * called by ImageInpainting::process (image_inpainting.cpp:125-153): fills the masked

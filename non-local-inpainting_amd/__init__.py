@@ -42,6 +42,7 @@ EXPORTS = [
     "nli_upsample", "nli_extended_domain", "nli_scale_masks", "nli_scale_nnf", "nli_patch_distance",
     "nli_patchmatch", "nli_patchmatch_dev", "nli_update", "nli_update_dev", "nli_poisson_fill",
     "nli_inpaint", "nli_inpaint_dev", "nli_inpaint_params_default", "nli_profile_enable",
+    "nli_patch_energy", "nli_energy_gradient", "nli_patch_extract", "nli_adjoint_aggregate",
     "nli_profile_read", "nli_inpaint_batch", "nli_inpaint_batch_dev",
 ]
 # symbols of libnli_plugins.so (include/nli_plugins.hpp)
@@ -88,6 +89,10 @@ class InpaintParams(ct.Structure):
                 ("init", ct.c_int), ("pm_iters", ct.c_int), ("pm_shots", ct.c_int),
                 ("pm_window", ct.c_int), ("cg_tol", ct.c_float), ("cg_limit", ct.c_int),
                 ("cg_precision", ct.c_int), ("cg_refine_every", ct.c_int), ("seed", ct.c_uint64)]
+
+
+class EnergyParams(ct.Structure):
+    _fields_ = [("patch", ct.c_int), ("sigma", ct.c_float), ("lambda_", ct.c_float)]
 
 
 class InpaintStats(ct.Structure):
@@ -408,6 +413,63 @@ class Context:
         out = np.empty(shape, dtype)
         self._check(self.lib.nli_copy_d2h(self.h, out.ctypes.data_as(ct.c_void_p), ct.c_void_p(p),
                                           ct.c_uint64(out.nbytes)))
+        return out
+
+    # ---- the NL-Poisson patch energy and its adjoint ----
+    def patch_energy(self, image, ext, nnf, conf=None, patch=9, sigma=10000.0, lam=0.05, touch=None, delta=0.0):
+        """E(u) for a fixed NNF; touch=(x, y, c) sums only the terms reading that pixel,
+        evaluated with its value + delta (FP64)."""
+        image = _c(image, np.float32)
+        H, W, C = image.shape
+        ext = _c(ext, np.uint8)
+        nnf = _c(nnf, np.int32)
+        conf = None if conf is None else _c(conf, np.float32)
+        px, py, pc = (int(v) for v in touch) if touch is not None else (-1, -1, -1)
+        e = ct.c_double()
+        p = EnergyParams(patch, sigma, lam)
+        self._check(self.lib.nli_patch_energy(self.h, ct.byref(p), _ptr(image, _f32p), _ptr(ext, _u8p),
+                                              _ptr(nnf, _i32p), _ptr(conf, _f32p), W, H, C, px, py, pc,
+                                              ct.c_double(delta), ct.byref(e)))
+        return e.value
+
+    def energy_gradient(self, image, ext, nnf, conf=None, patch=9, sigma=10000.0, lam=0.05):
+        image = _c(image, np.float32)
+        H, W, C = image.shape
+        ext = _c(ext, np.uint8)
+        nnf = _c(nnf, np.int32)
+        conf = None if conf is None else _c(conf, np.float32)
+        g = np.zeros((H, W, C), np.float64)
+        p = EnergyParams(patch, sigma, lam)
+        self._check(self.lib.nli_energy_gradient(self.h, ct.byref(p), _ptr(image, _f32p), _ptr(ext, _u8p),
+                                                 _ptr(nnf, _i32p), _ptr(conf, _f32p), W, H, C,
+                                                 g.ctypes.data_as(ct.POINTER(ct.c_double))))
+        return g
+
+    def patch_extract(self, u, ext, nnf, patch=9, side=0):
+        u = _c(u, np.float64)
+        H, W, C = u.shape
+        ext = _c(ext, np.uint8)
+        nnf = _c(nnf, np.int32)
+        ne = int(ext.astype(bool).sum())
+        out = np.zeros((ne, patch * patch, C), np.float64)
+        n = ct.c_int()
+        self._check(self.lib.nli_patch_extract(self.h, patch, u.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                               _ptr(ext, _u8p), _ptr(nnf, _i32p), W, H, C, side,
+                                               out.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                               ct.c_uint64(out.size), ct.byref(n)))
+        assert n.value == ne
+        return out
+
+    def adjoint_aggregate(self, patches, ext, nnf, patch=9, side=0):
+        patches = _c(patches, np.float64)
+        ext = _c(ext, np.uint8)
+        nnf = _c(nnf, np.int32)
+        H, W = ext.shape
+        C = patches.shape[-1]
+        out = np.zeros((H, W, C), np.float64)
+        self._check(self.lib.nli_adjoint_aggregate(self.h, patch, patches.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                                   patches.shape[0], _ptr(ext, _u8p), _ptr(nnf, _i32p), W, H, C,
+                                                   side, out.ctypes.data_as(ct.POINTER(ct.c_double))))
         return out
 
     def copy_d2d(self, dst: int, src: int, nbytes: int):

@@ -42,6 +42,8 @@ struct nli_ctx {
     nli::DBuf<uint8_t> b0, b1, b2, b3;
     nli::DBuf<int32_t> n0, n1;
     nli::DBuf<int> pts, counts, total, nonempty;
+    nli::DBuf<double> d0, d1;
+    nli::AdjWork adj;
     std::vector<std::unique_ptr<nli_worker>> workers;   // batch workers, created on demand
 };
 
@@ -579,6 +581,91 @@ int nli_scale_masks(nli_ctx *ctx, const uint8_t *hole, int W, int H, int patch, 
         return NLI_E_EMPTY_SOURCE;
     }
     return rc;
+}
+
+namespace {
+
+nli::EnergyDesc edesc(const nli_energy_params *e)
+{
+    need(e != nullptr, "null energy parameters");
+    need(e->patch >= 1 && e->patch <= 15 && e->patch % 2 == 1, "patch side must be odd and at most 15");
+    return nli::EnergyDesc{e->patch, e->sigma, e->lambda};
+}
+
+}  // namespace
+
+int nli_patch_energy(nli_ctx *ctx, const nli_energy_params *params, const float *image, const uint8_t *ext,
+                     const int32_t *nnf, const float *conf, int W, int H, int C, int px, int py, int pc, double delta,
+                     double *energy)
+{
+    return guard(ctx, [&] {
+        const nli::EnergyDesc d = edesc(params);
+        need(image && ext && nnf && energy && W > 0 && H > 0 && C >= 1 && C <= 4, "bad energy arguments");
+        const size_t n = (size_t)W * H;
+        const float *d_i = stage_in(ctx, ctx->f0, image, n * C);
+        const uint8_t *d_e = stage_in(ctx, ctx->b0, ext, n);
+        const int32_t *d_n = stage_in(ctx, ctx->n0, nnf, 2 * n);
+        const float *d_c = conf ? stage_in(ctx, ctx->f1, conf, n) : nullptr;
+        *energy = nli::patch_energy(d, d_i, d_e, d_n, d_c, W, H, C, px, py, pc, delta, ctx->adj, ctx->stream);
+    });
+}
+
+int nli_energy_gradient(nli_ctx *ctx, const nli_energy_params *params, const float *image, const uint8_t *ext,
+                        const int32_t *nnf, const float *conf, int W, int H, int C, double *grad)
+{
+    return guard(ctx, [&] {
+        const nli::EnergyDesc d = edesc(params);
+        need(image && ext && nnf && grad && W > 0 && H > 0 && C >= 1 && C <= 4, "bad gradient arguments");
+        const size_t n = (size_t)W * H;
+        const float *d_i = stage_in(ctx, ctx->f0, image, n * C);
+        const uint8_t *d_e = stage_in(ctx, ctx->b0, ext, n);
+        const int32_t *d_n = stage_in(ctx, ctx->n0, nnf, 2 * n);
+        const float *d_c = conf ? stage_in(ctx, ctx->f1, conf, n) : nullptr;
+        double *d_g = ctx->d0.ensure(n * C);
+        nli::energy_gradient(d, d_i, d_e, d_n, d_c, W, H, C, d_g, ctx->adj, ctx->stream);
+        stage_out(ctx, grad, d_g, n * C);
+        sync(ctx);
+    });
+}
+
+int nli_patch_extract(nli_ctx *ctx, int patch, const double *u, const uint8_t *ext, const int32_t *nnf, int W,
+                      int H, int C, int side, double *patches, uint64_t capacity, int *n_centres)
+{
+    return guard(ctx, [&] {
+        need(u && ext && nnf && patches && n_centres && W > 0 && H > 0 && C >= 1 && C <= 4, "bad extract arguments");
+        need(side == 0 || side == 1, "side must be 0 (target) or 1 (source)");
+        const size_t n = (size_t)W * H;
+        const double *d_u = stage_in(ctx, ctx->d0, u, n * C);
+        const uint8_t *d_e = stage_in(ctx, ctx->b0, ext, n);
+        const int32_t *d_n = stage_in(ctx, ctx->n0, nnf, 2 * n);
+        double *d_p = ctx->d1.ensure(std::max<size_t>((size_t)capacity, 1));
+        const int nc = nli::patch_extract(patch, d_u, d_e, d_n, W, H, C, side, d_p, (size_t)capacity, ctx->adj,
+                                          ctx->stream);
+        stage_out(ctx, patches, d_p, (size_t)nc * patch * patch * C);
+        sync(ctx);
+        *n_centres = nc;
+    });
+}
+
+int nli_adjoint_aggregate(nli_ctx *ctx, int patch, const double *patches, int n_centres, const uint8_t *ext,
+                          const int32_t *nnf, int W, int H, int C, int side, double *out)
+{
+    return guard(ctx, [&] {
+        need(patches && ext && nnf && out && W > 0 && H > 0 && C >= 1 && C <= 4 && n_centres >= 0,
+             "bad aggregate arguments");
+        need(side == 0 || side == 1, "side must be 0 (target) or 1 (source)");
+        const size_t n = (size_t)W * H;
+        const uint8_t *d_e = stage_in(ctx, ctx->b0, ext, n);
+        const int32_t *d_n = stage_in(ctx, ctx->n0, nnf, 2 * n);
+        size_t ne = 0;
+        for (size_t i = 0; i < n; i++) ne += ext[i] != 0;
+        need(ne == (size_t)n_centres, "n_centres differs from the number of extended-domain points");
+        const double *d_p = stage_in(ctx, ctx->d1, patches, (size_t)n_centres * patch * patch * C);
+        double *d_o = ctx->d0.ensure(n * C);
+        nli::adjoint_aggregate(patch, d_p, d_e, d_n, W, H, C, side, d_o, ctx->adj, ctx->stream);
+        stage_out(ctx, out, d_o, n * C);
+        sync(ctx);
+    });
 }
 
 int nli_patch_distance(nli_ctx *ctx, const nli_dist_params *dist, const float *src, int sW, int sH, const float *tgt,

@@ -279,4 +279,33 @@ struct DriverWork {
 void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask, int W, int H, int C,
              float *d_out, InpaintReport &rep, DriverWork &dw, PMWork &pmw, UpdWork &uw, hipStream_t s);
 
+// ---- patch energy, its gradient, extraction/aggregation pair (k_adjoint.hip) ----
+struct EnergyDesc {
+    int patch;
+    float sigma;
+    float lambda;
+};
+struct AdjWork {
+    DBuf<float> weights;
+    int w_patch = -1;
+    float w_sigma = 0.0f;
+    DBuf<int> pts, lidx, counts, total;
+    DBuf<double> partials, S, Rx, Ry;
+};
+// energy over the extended-domain centres; with tx >= 0, only the terms that read pixel
+// (tx, ty) channel tc, whose value is taken as img + delta (FP64 finite differences)
+double patch_energy(const EnergyDesc &d, const float *img, const uint8_t *ext, const int32_t *nnf,
+                    const float *conf, int W, int H, int C, int tx, int ty, int tc, double delta, AdjWork &w,
+                    hipStream_t s);
+// dE/du for every pixel and channel (HWC, FP64)
+void energy_gradient(const EnergyDesc &d, const float *img, const uint8_t *ext, const int32_t *nnf,
+                     const float *conf, int W, int H, int C, double *grad, AdjWork &w, hipStream_t s);
+// side 0: u(p + h), side 1: u(nnf(p) + h) for the extended-domain centres p in row-major
+// order; returns the number of centres
+int patch_extract(int P, const double *u, const uint8_t *ext, const int32_t *nnf, int W, int H, int C, int side,
+                  double *out, size_t out_capacity, AdjWork &w, hipStream_t s);
+// the adjoint of patch_extract (same side)
+void adjoint_aggregate(int P, const double *patches, const uint8_t *ext, const int32_t *nnf, int W, int H, int C,
+                       int side, double *out, AdjWork &w, hipStream_t s);
+
 }  // namespace nli
