@@ -121,12 +121,6 @@ struct TgtGlobal {
     }
 };
 
-// Gradient distances (L2Combined with C = 1 or 3, L2RGBD) for a compile-time patch
-// side: the source gradient is formed on the fly from the image rows the colour term
-// loads anyway (forward differences, zero past the last column/row, the exact float
-// subtractions of gradient.cpp:32-73), so a source patch costs (P+1)^2 C floats of
-// gathers instead of 3 P^2 C.  Arithmetic and summation order are those of
-// patch_distance below, so results are identical.
 // One row (dy) of a gradient distance: cur = image row sy - r + dy, nxt = the row below
 // (the same row again at the bottom image border, so its vertical differences are the
 // exact +0 of gradient.cpp:49-52); column P of both is column x0 + P, or column
@@ -467,7 +461,8 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
     int2 best = a.nnf_in[p];
     float bd = a.dist_in[p];
 
-    // phase 1: propagation
+    // phase 1: propagation.  A candidate equal to the current best is skipped: its
+    // distance is exactly bd, so the reference's strict comparison never takes it.
     {
         const int s = a.step;
         const int ox = lane == 0 ? -s : (lane == 1 ? s : 0);
@@ -480,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
             int2 e = a.nnf_in[(size_t)a.tW * qy + qx];
             e.x -= ox;
             e.y -= oy;
-            if (is_source(a, e.x, e.y)) {
+            if ((e.x != best.x || e.y != best.y) && is_source(a, e.x, e.y)) {
                 d = patch_distance<KIND, C, PS>(a, wl, e.x, e.y, t, bd);
                 c = e;
                 order = lane;
@@ -516,6 +511,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
                     int cx = x_min + rng.below(x_max - x_min);
                     int cy = y_min + rng.below(y_max - y_min);
                     if (is_source(a, cx, cy)) {
+                        if (cx == best.x && cy == best.y) break;   // the phase-1 best: never strictly better
                         const float bound = fminf(bd, gd);
                         float dd = patch_distance<KIND, C, PS>(a, wl, cx, cy, t, bound);
                         if (dd < bound) {
@@ -540,6 +536,256 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
         }
     }
     if (lane == 0) {
+        a.nnf_out[p] = best;
+        a.dist_out[p] = bd;
+    }
+}
+
+// ---- sweep with a per-wave candidate pool (gradient distances, compile-time patch) ----
+// Phase 1 is the propagation round of k_pm_sweep.  For phase 2 each wave (16 targets)
+// first draws the candidate of every (target, window) pair -- the reference's random
+// search, windows max(W,H)/2^k around the phase-1 best, at most `shots` tries, the first
+// in-source draw kept -- into LDS, then evaluates them as a pool: a lane whose candidate
+// is rejected by early termination takes the next one, so the wave no longer waits for
+// its slowest lane per window.  Each target's best (distance, window) is an LDS 64-bit
+// atomic minimum; a candidate is pruned only when its partial sum exceeds the target's
+// current best, which never drops below the final minimum, and ties go to the lower
+// window index -- the result equals the reference's sequential acceptance (strictly
+// better than the phase-1 best, first window on ties) over the same candidates.
+constexpr int kMaxR = 16;                 // random-search windows (max(W, H) < 65536)
+constexpr int kWaves = kBlock / 64;
+constexpr int kWaveTargets = 64 / kLanes; // targets per wave (one tile row)
+static_assert(kWaveTargets == TX, "a wave covers one row of the target tile");
+
+__device__ inline int lane_rank(uint64_t mask)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+// Orders this wave's LDS writes before its later LDS reads (the pool tables are private
+// to a wave, so no workgroup barrier is needed).
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+__device__ inline uint64_t pool_key(float d, int wi) { return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)wi; }
+__device__ inline float key_dist(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+
+struct PoolLds {
+    uint64_t *best;    // [kWaves][kWaveTargets]
+    int *cand;         // [kWaves][kWaveTargets * kMaxR]: cx | cy << 16, or -1
+    int2 *best1;       // [kWaves][kWaveTargets]: phase-1 best
+    float *bd1;        // [kWaves][kWaveTargets]: phase-1 distance (-1: no target)
+    uint8_t *list;     // [kWaves][kWaveTargets * kMaxR]: drawn tasks in draw order
+};
+__host__ __device__ inline size_t pool_lds_offset(int F, int P)
+{
+    const int r = P / 2;
+    const size_t base = ((size_t)F * (TX + 2 * r) * (TY + 2 * r) + (size_t)P * P) * sizeof(float);
+    return (base + 15) & ~(size_t)15;
+}
+__host__ __device__ inline size_t pool_lds_bytes()
+{
+    return (size_t)kWaves * kWaveTargets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)) +
+           (size_t)kWaves * kWaveTargets * kMaxR * (sizeof(int) + sizeof(uint8_t));
+}
+
+template <int KIND, int C, int PS>
+__global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
+{
+    static_assert(PS > 0 && (KIND == DIST_L2RGBD || (KIND == DIST_L2COMBINED && (C == 1 || C == 3))),
+                  "pool sweep: gradient distances with a compile-time patch");
+    extern __shared__ float lds[];
+    constexpr int F = 3 * C;
+    constexpr int P = PS, r = PS / 2;
+    constexpr unsigned PP = (unsigned)(P * P);
+    constexpr int LW = TX + 2 * r, LH = TY + 2 * r;
+    constexpr int plane = LW * LH;
+    float *wl = lds + F * plane;
+    char *pb = reinterpret_cast<char *>(lds) + pool_lds_offset(F, P);
+    PoolLds pl;
+    pl.best = reinterpret_cast<uint64_t *>(pb);
+    pl.best1 = reinterpret_cast<int2 *>(pb + kWaves * kWaveTargets * sizeof(uint64_t));
+    pl.bd1 = reinterpret_cast<float *>(pb + kWaves * kWaveTargets * (sizeof(uint64_t) + sizeof(int2)));
+    pl.cand = reinterpret_cast<int *>(pb + kWaves * kWaveTargets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)));
+    pl.list = reinterpret_cast<uint8_t *>(pl.cand + kWaves * kWaveTargets * kMaxR);
+
+    const int per = (int)(gridDim.x / kXCDs);
+    const int slot = (int)(blockIdx.x % kXCDs) * per + (int)(blockIdx.x / kXCDs);
+    if (slot >= a.ntiles) return;   // whole block: uniform
+    const int tile = a.tiles[slot];
+    const int x0 = (tile % a.ntiles_x) * TX, y0 = (tile / a.ntiles_x) * TY;
+    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = a.w[i];
+    for (int i = threadIdx.x; i < plane; i += blockDim.x) {
+        int gx = x0 - r + i % LW, gy = y0 - r + i / LW;
+        bool in = (unsigned)gx < (unsigned)a.tW && (unsigned)gy < (unsigned)a.tH;
+        size_t q = in ? (size_t)gy * a.tW + gx : 0;
+#pragma unroll
+        for (int c = 0; c < C; c++) lds[c * plane + i] = in ? a.tgt[q * C + c] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 2 * C; k++) lds[(C + k) * plane + i] = in ? a.tgrad[q * 2 * C + k] : 0.0f;
+    }
+    __syncthreads();
+    const int lane = (int)threadIdx.x & (kLanes - 1);
+    const int wave = (int)threadIdx.x / 64;
+    const int lw = (int)threadIdx.x & 63;
+    const int lx = lw / kLanes;   // target of this lane's group within the wave
+    const int x = x0 + lx, y = y0 + wave;
+    // no early exit below: the wave-level steps need every lane
+    const bool valid = x < a.tW && y < a.tH && is_target(a, x, y);
+    const size_t p = valid ? (size_t)a.tW * y + x : 0;
+    int2 best = valid ? a.nnf_in[p] : make_int2(-1, -1);
+    float bd = valid ? a.dist_in[p] : INFINITY;
+
+    // phase 1: propagation (as k_pm_sweep)
+    {
+        const int st = a.step;
+        const int ox = lane == 0 ? -st : (lane == 1 ? st : 0);
+        const int oy = lane == 2 ? -st : (lane == 3 ? st : 0);
+        float d = INFINITY;
+        int2 c = best;
+        int order = INT_MAX;
+        const int qx = x + ox, qy = y + oy;
+        if (valid && is_target(a, qx, qy)) {
+            int2 e = a.nnf_in[(size_t)a.tW * qy + qx];
+            e.x -= ox;
+            e.y -= oy;
+            if ((e.x != best.x || e.y != best.y) && is_source(a, e.x, e.y)) {
+                const TgtLds t{lds, plane, LW, lx, wave};
+                d = patch_distance_fly<KIND, C, PS>(a, wl, e.x, e.y, t, bd);
+                c = e;
+                order = lane;
+            }
+        }
+        group_min(d, c, order);
+        if (d < bd) {
+            bd = d;
+            best = c;
+        }
+    }
+    if (lane == 0) {
+        pl.best1[wave * kWaveTargets + lx] = best;
+        pl.bd1[wave * kWaveTargets + lx] = valid ? bd : -1.0f;
+        pl.best[wave * kWaveTargets + lx] = pool_key(INFINITY, 0x7FFFFFFF);
+    }
+    wave_sync();
+
+    // phase 2a: draw the candidate of every (target, window) of the wave
+    int R = 0;
+    for (int ws = a.maxw; ws >= 1; ws /= 2) R++;
+    const int ntask = kWaveTargets * R;   // task j: target j % 16, window j / 16
+    int *cand = pl.cand + wave * kWaveTargets * kMaxR;
+    uint8_t *list = pl.list + wave * kWaveTargets * kMaxR;
+    int nvalid = 0;
+    for (int base = 0; base < ntask; base += 64) {
+        const int j = base + lw;
+        bool ok = false;
+        if (j < ntask) {
+            const int t = j % kWaveTargets, wi = j / kWaveTargets;
+            int packed = -1;
+            if (pl.bd1[wave * kWaveTargets + t] >= 0.0f) {
+                const int2 ctr = pl.best1[wave * kWaveTargets + t];
+                const uint32_t pt = (uint32_t)((size_t)a.tW * (y0 + wave) + x0 + t);
+                Rng rng = make_rng(a.seed, a.stream * 32u + (uint32_t)wi, pt);
+                const int ws = a.maxw >> wi;
+                const int x_min = max(ctr.x - ws, 0), y_min = max(ctr.y - ws, 0);
+                const int x_max = min(ctr.x + ws + 1, a.sW), y_max = min(ctr.y + ws + 1, a.sH);
+                for (int k = 0; k < a.shots; k++) {
+                    const int cx = x_min + rng.below(x_max - x_min);
+                    const int cy = y_min + rng.below(y_max - y_min);
+                    if (is_source(a, cx, cy)) {
+                        // the phase-1 best itself is never strictly better: no evaluation
+                        ok = cx != ctr.x || cy != ctr.y;
+                        if (ok) packed = cx | (cy << 16);
+                        break;
+                    }
+                }
+            }
+            cand[j] = packed;
+        }
+        const uint64_t m = __ballot(ok);
+        if (ok) list[nvalid + lane_rank(m)] = (uint8_t)j;
+        nvalid += __popcll(m);
+    }
+    wave_sync();
+
+    // phase 2b: evaluate the pool, smallest windows first
+    const double lam = (double)a.lambda, lam1 = (double)(1 - a.lambda);
+    const double lamd = (double)a.lambda_d, lamd1 = (double)(1 - a.lambda_d);
+    const size_t stride = (size_t)a.sW * C;
+    uint64_t *bestk = pl.best + wave * kWaveTargets;
+    const float *bd1 = pl.bd1 + wave * kWaveTargets;
+    bool active = false;
+    int t = 0, wi = 0, cy = 0, dy = 0, last = P;
+    float bound = 0.0f;
+    const float *row = a.src;
+    double dist = 0.0, gdist = 0.0, dist_d = 0.0, gdist_d = 0.0;
+    float A[P + 1][C], B[P + 1][C];
+    auto partial = [&]() {
+        double v = lam * dist + lam1 * gdist;
+        if constexpr (KIND == DIST_L2RGBD) v = v + lamd * dist_d + lamd1 * gdist_d;
+        return v;
+    };
+    auto finish = [&]() {
+        const float d = (float)(partial() / (double)PP);
+        if (d <= bound) atomicMin((unsigned long long *)&bestk[t], (unsigned long long)pool_key(d, wi));
+    };
+    int next = 0;
+    for (;;) {
+        const bool need = !active;
+        const uint64_t m = __ballot(need);
+        if (need) {
+            const int k = next + lane_rank(m);
+            if (k < nvalid) {
+                const int j = list[nvalid - 1 - k];
+                t = j % kWaveTargets;
+                wi = j / kWaveTargets;
+                const int packed = cand[j];
+                const int cx = packed & 0xFFFF;
+                cy = packed >> 16;
+                bound = fminf(bd1[t], key_dist(bestk[t]));
+                dy = 0;
+                dist = gdist = dist_d = gdist_d = 0.0;
+                last = (cx + r + 1 < a.sW) ? P : P - 1;
+                row = a.src + ((size_t)(cy - r) * a.sW + (cx - r)) * C;
+                load_row<C, P>(A, row, last);
+                active = true;
+            }
+        }
+        next += __popcll(m);
+        if (__ballot(active) == 0) break;
+        if (active) {
+            // one patch row per pass: row dy is in A, row dy + 1 goes to B, then B -> A
+            bound = fminf(bound, key_dist(bestk[t]));
+            const TgtLds tg{lds, plane, LW, t, wave};
+            const float *nrow = (cy - r + dy + 1 < a.sH) ? row + stride : row;
+            load_row<C, P>(B, nrow, last);
+            fly_row<KIND, C, P>(A, B, wl, tg, dy, dist, gdist, dist_d, gdist_d);
+            if (partial() > (double)bound * (double)PP * (1.0 + 1e-6)) {
+                active = false;
+            } else if (dy + 1 >= P) {
+                finish();
+                active = false;
+            } else {
+#pragma unroll
+                for (int i = 0; i <= P; i++)
+#pragma unroll
+                    for (int c = 0; c < C; c++) A[i][c] = B[i][c];
+                row = nrow;
+                dy++;
+            }
+        }
+    }
+    wave_sync();
+    if (valid && lane == 0) {
+        const uint64_t k = bestk[lx];
+        const float gd = key_dist(k);
+        if (gd < bd) {
+            const int packed = cand[(int)(uint32_t)k * kWaveTargets + lx];
+            best = make_int2(packed & 0xFFFF, packed >> 16);
+            bd = gd;
+        }
         a.nnf_out[p] = best;
         a.dist_out[p] = bd;
     }
@@ -779,6 +1025,11 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     for (int ws = a.maxw; ws >= 1; ws /= 2) R++;
     const double per_pixel = (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) ? 9.0 * C + 4 : 3.0 * C + 2;
     const double sweep_flops = (double)pr.targets * (2 + R) * d.patch * d.patch * per_pixel;
+    // candidate-pool sweep for the specialised gradient distances (up to kMaxR windows)
+    const bool pool = R <= kMaxR && ((d.kind == DIST_L2COMBINED && C == 3 &&
+                                      (d.patch == 7 || d.patch == 9 || d.patch == 11)) ||
+                                     (d.kind == DIST_L2RGBD && C == 4 && d.patch == 9));
+    const size_t lds_pool = pool_lds_offset(F, d.patch) + pool_lds_bytes();
     for (int k = 0; k < K; k++) {
         a.nnf_in = n_cur;
         a.dist_in = d_cur;
@@ -787,9 +1038,20 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
         a.step = s0 >> (k % (levels + 1));
         a.stream = (uint32_t)k;
         prof::begin(TK_PM_SWEEP, s);
+        if (pool) {
+            if (d.kind == DIST_L2RGBD)
+                k_pm_sweep_pool<DIST_L2RGBD, 4, 9><<<sweep_blocks, kBlock, lds_pool, s>>>(a);
+            else if (d.patch == 7)
+                k_pm_sweep_pool<DIST_L2COMBINED, 3, 7><<<sweep_blocks, kBlock, lds_pool, s>>>(a);
+            else if (d.patch == 9)
+                k_pm_sweep_pool<DIST_L2COMBINED, 3, 9><<<sweep_blocks, kBlock, lds_pool, s>>>(a);
+            else
+                k_pm_sweep_pool<DIST_L2COMBINED, 3, 11><<<sweep_blocks, kBlock, lds_pool, s>>>(a);
+        } else {
 #define NLI_SWEEP(KD, CC, PP) k_pm_sweep<KD, CC, PP><<<sweep_blocks, kBlock, lds, s>>>(a)
-        NLI_DISPATCH(d.kind, C, d.patch, NLI_SWEEP);
+            NLI_DISPATCH(d.kind, C, d.patch, NLI_SWEEP);
 #undef NLI_SWEEP
+        }
         NLI_LAUNCH_CHECK();
         prof::end(TK_PM_SWEEP, s, sweep_flops, pr.targets);
         std::swap(n_cur, n_oth);
