@@ -579,12 +579,23 @@ struct PoolLds {
     float *bd1;        // [kWaves][kWaveTargets]: phase-1 distance (-1: no target)
     uint8_t *list;     // [kWaves][kWaveTargets * kMaxR]: drawn tasks in draw order
 };
+// the pool kernel keeps the target tile pixel-interleaved: F features padded to a
+// multiple of 4 floats per pixel, so one pixel's features are two or three aligned
+// 16-byte LDS reads at immediate offsets
+__host__ __device__ constexpr int pool_feat_stride(int F) { return (F + 3) & ~3; }
 __host__ __device__ inline size_t pool_lds_offset(int F, int P)
 {
     const int r = P / 2;
-    const size_t base = ((size_t)F * (TX + 2 * r) * (TY + 2 * r) + (size_t)P * P) * sizeof(float);
+    const size_t base =
+        ((size_t)pool_feat_stride(F) * (TX + 2 * r) * (TY + 2 * r) + (size_t)P * P) * sizeof(float);
     return (base + 15) & ~(size_t)15;
 }
+template <int FP>
+struct TgtLdsI {
+    const float *lds;
+    int LW, ox, oy;
+    __device__ float v(int f, int dx, int dy) const { return lds[((oy + dy) * LW + ox + dx) * FP + f]; }
+};
 __host__ __device__ inline size_t pool_lds_bytes()
 {
     return (size_t)kWaves * kWaveTargets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)) +
@@ -596,13 +607,15 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
 {
     static_assert(PS > 0 && (KIND == DIST_L2RGBD || (KIND == DIST_L2COMBINED && (C == 1 || C == 3))),
                   "pool sweep: gradient distances with a compile-time patch");
-    extern __shared__ float lds[];
+    extern __shared__ __align__(16) float plds[];
+    float *lds = plds;
     constexpr int F = 3 * C;
+    constexpr int FP = pool_feat_stride(F);
     constexpr int P = PS, r = PS / 2;
     constexpr unsigned PP = (unsigned)(P * P);
     constexpr int LW = TX + 2 * r, LH = TY + 2 * r;
     constexpr int plane = LW * LH;
-    float *wl = lds + F * plane;
+    float *wl = lds + FP * plane;
     char *pb = reinterpret_cast<char *>(lds) + pool_lds_offset(F, P);
     PoolLds pl;
     pl.best = reinterpret_cast<uint64_t *>(pb);
@@ -621,10 +634,13 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
         int gx = x0 - r + i % LW, gy = y0 - r + i / LW;
         bool in = (unsigned)gx < (unsigned)a.tW && (unsigned)gy < (unsigned)a.tH;
         size_t q = in ? (size_t)gy * a.tW + gx : 0;
+        float *px = lds + (size_t)i * FP;
 #pragma unroll
-        for (int c = 0; c < C; c++) lds[c * plane + i] = in ? a.tgt[q * C + c] : 0.0f;
+        for (int c = 0; c < C; c++) px[c] = in ? a.tgt[q * C + c] : 0.0f;
 #pragma unroll
-        for (int k = 0; k < 2 * C; k++) lds[(C + k) * plane + i] = in ? a.tgrad[q * 2 * C + k] : 0.0f;
+        for (int k = 0; k < 2 * C; k++) px[C + k] = in ? a.tgrad[q * 2 * C + k] : 0.0f;
+#pragma unroll
+        for (int k = F; k < FP; k++) px[k] = 0.0f;
     }
     __syncthreads();
     const int lane = (int)threadIdx.x & (kLanes - 1);
@@ -652,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
             e.x -= ox;
             e.y -= oy;
             if ((e.x != best.x || e.y != best.y) && is_source(a, e.x, e.y)) {
-                const TgtLds t{lds, plane, LW, lx, wave};
+                const TgtLdsI<FP> t{lds, LW, lx, wave};
                 d = patch_distance_fly<KIND, C, PS>(a, wl, e.x, e.y, t, bd);
                 c = e;
                 order = lane;
@@ -758,7 +774,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
         if (active) {
             // one patch row per pass: row dy is in A, row dy + 1 goes to B, then B -> A
             bound = fminf(bound, key_dist(bestk[t]));
-            const TgtLds tg{lds, plane, LW, t, wave};
+            const TgtLdsI<FP> tg{lds, LW, t, wave};
             const float *nrow = (cy - r + dy + 1 < a.sH) ? row + stride : row;
             load_row<C, P>(B, nrow, last);
             fly_row<KIND, C, P>(A, B, wl, tg, dy, dist, gdist, dist_d, gdist_d);
