@@ -541,21 +541,24 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
     }
 }
 
-// ---- sweep with a per-wave candidate pool (gradient distances, compile-time patch) ----
-// Phase 1 is the propagation round of k_pm_sweep.  For phase 2 each wave (16 targets)
-// first draws the candidate of every (target, window) pair -- the reference's random
-// search, windows max(W,H)/2^k around the phase-1 best, at most `shots` tries, the first
-// in-source draw kept -- into LDS, then evaluates them as a pool: a lane whose candidate
-// is rejected by early termination takes the next one, so the wave no longer waits for
-// its slowest lane per window.  Each target's best (distance, window) is an LDS 64-bit
-// atomic minimum; a candidate is pruned only when its partial sum exceeds the target's
-// current best, which never drops below the final minimum, and ties go to the lower
-// window index -- the result equals the reference's sequential acceptance (strictly
-// better than the phase-1 best, first window on ties) over the same candidates.
-constexpr int kMaxR = 16;                 // random-search windows (max(W, H) < 65536)
+// ---- sweep with per-wave candidate pools (gradient distances, compile-time patch) ----
+// A block takes two target tiles; wave w takes row w of both (32 targets).  The sweep's
+// candidates -- the 4 jump-flood neighbours' shifted matches, then, around the
+// resulting best, the reference's random search (windows max(W,H)/2^k, at most `shots`
+// tries, the first in-source draw kept) -- are first written to LDS per wave and then
+// evaluated as a pool: a lane whose candidate is pruned by early termination takes the
+// next one, so lanes do not wait for the slowest candidate of a round.  A candidate equal
+// to the current best is dropped: its distance is exactly the current one and the
+// reference's strict comparison never takes it.  Each target's best (distance, order) is
+// an LDS 64-bit atomic minimum; a candidate is pruned only when its partial sum exceeds
+// the target's current best, which never drops below the final minimum, and ties go to
+// the lower order -- the result equals the sequential strict-improvement acceptance over
+// the same candidates (neighbours in order left, right, up, down; windows largest first).
+constexpr int kMaxR = 16;                      // random-search windows (max(W, H) < 65536)
 constexpr int kWaves = kBlock / 64;
-constexpr int kWaveTargets = 64 / kLanes; // targets per wave (one tile row)
-static_assert(kWaveTargets == TX, "a wave covers one row of the target tile");
+constexpr int kTilesPerBlock = 1;
+constexpr int kWaveTargets = kTilesPerBlock * TX;   // targets per wave
+static_assert(kWaves == TY, "wave w covers row w of each tile");
 
 __device__ inline int lane_rank(uint64_t mask)
 {
@@ -569,26 +572,29 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
-__device__ inline uint64_t pool_key(float d, int wi) { return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)wi; }
+__device__ inline uint64_t pool_key(float d, int order)
+{
+    return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)order;
+}
 __device__ inline float key_dist(uint64_t k) { return __uint_as_float((uint32_t)(k >> 32)); }
+constexpr uint64_t kNoKey = ((uint64_t)0x7F800000u << 32) | 0xFFFFFFFFu;   // (+inf, max)
 
-struct PoolLds {
-    uint64_t *best;    // [kWaves][kWaveTargets]
-    int *cand;         // [kWaves][kWaveTargets * kMaxR]: cx | cy << 16, or -1
-    int2 *best1;       // [kWaves][kWaveTargets]: phase-1 best
-    float *bd1;        // [kWaves][kWaveTargets]: phase-1 distance (-1: no target)
-    uint8_t *list;     // [kWaves][kWaveTargets * kMaxR]: drawn tasks in draw order
-};
-// the pool kernel keeps the target tile pixel-interleaved: F features padded to a
-// multiple of 4 floats per pixel, so one pixel's features are two or three aligned
-// 16-byte LDS reads at immediate offsets
+// the target tiles are kept pixel-interleaved: F features padded to a multiple of 4
+// floats per pixel, so one pixel's features are 16-byte LDS reads at immediate offsets
 __host__ __device__ constexpr int pool_feat_stride(int F) { return (F + 3) & ~3; }
-__host__ __device__ inline size_t pool_lds_offset(int F, int P)
+__host__ __device__ inline int pool_tasks_per_target(int R) { return R > 4 ? R : 4; }
+__host__ __device__ inline size_t pool_tables_offset(int F, int P)
 {
     const int r = P / 2;
-    const size_t base =
-        ((size_t)pool_feat_stride(F) * (TX + 2 * r) * (TY + 2 * r) + (size_t)P * P) * sizeof(float);
-    return (base + 15) & ~(size_t)15;
+    const size_t tiles = (size_t)kTilesPerBlock * pool_feat_stride(F) * (TX + 2 * r) * (TY + 2 * r);
+    return ((tiles + (size_t)P * P) * sizeof(float) + 15) & ~(size_t)15;
+}
+// per wave: keys (u64), best (int2), distance (float) per target; candidate (int) and
+// list entry (u16) per task
+__host__ __device__ inline size_t pool_wave_bytes(int R)
+{
+    return (size_t)kWaveTargets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)) +
+           (size_t)kWaveTargets * pool_tasks_per_target(R) * (sizeof(int) + sizeof(uint16_t));
 }
 template <int FP>
 struct TgtLdsI {
@@ -596,144 +602,28 @@ struct TgtLdsI {
     int LW, ox, oy;
     __device__ float v(int f, int dx, int dy) const { return lds[((oy + dy) * LW + ox + dx) * FP + f]; }
 };
-__host__ __device__ inline size_t pool_lds_bytes()
-{
-    return (size_t)kWaves * kWaveTargets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)) +
-           (size_t)kWaves * kWaveTargets * kMaxR * (sizeof(int) + sizeof(uint8_t));
-}
 
-template <int KIND, int C, int PS>
-__global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
+struct WavePool {
+    uint64_t *keys;   // [kWaveTargets] best (distance, order) of the running pool
+    int2 *best;       // [kWaveTargets] current best match
+    float *bd;        // [kWaveTargets] its distance (-1: not a target)
+    int *cand;        // [kWaveTargets * tasks]: cx | cy << 16 for task order * 32 + target
+    uint16_t *list;   // [kWaveTargets * tasks]: drawn tasks
+};
+
+// Evaluates the `nvalid` listed tasks of a wave (task j: target j % 32, order j / 32,
+// candidate cand[j]); records each completed candidate in keys[target] by atomic minimum.
+template <int KIND, int C, int P, int FP, bool LAST_FIRST>
+__device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const float *tiles, int LW, int tile_floats,
+                         int wave, const WavePool &wp, int nvalid)
 {
-    static_assert(PS > 0 && (KIND == DIST_L2RGBD || (KIND == DIST_L2COMBINED && (C == 1 || C == 3))),
-                  "pool sweep: gradient distances with a compile-time patch");
-    extern __shared__ __align__(16) float plds[];
-    float *lds = plds;
-    constexpr int F = 3 * C;
-    constexpr int FP = pool_feat_stride(F);
-    constexpr int P = PS, r = PS / 2;
+    constexpr int r = P / 2;
     constexpr unsigned PP = (unsigned)(P * P);
-    constexpr int LW = TX + 2 * r, LH = TY + 2 * r;
-    constexpr int plane = LW * LH;
-    float *wl = lds + FP * plane;
-    char *pb = reinterpret_cast<char *>(lds) + pool_lds_offset(F, P);
-    PoolLds pl;
-    pl.best = reinterpret_cast<uint64_t *>(pb);
-    pl.best1 = reinterpret_cast<int2 *>(pb + kWaves * kWaveTargets * sizeof(uint64_t));
-    pl.bd1 = reinterpret_cast<float *>(pb + kWaves * kWaveTargets * (sizeof(uint64_t) + sizeof(int2)));
-    pl.cand = reinterpret_cast<int *>(pb + kWaves * kWaveTargets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)));
-    pl.list = reinterpret_cast<uint8_t *>(pl.cand + kWaves * kWaveTargets * kMaxR);
-
-    const int per = (int)(gridDim.x / kXCDs);
-    const int slot = (int)(blockIdx.x % kXCDs) * per + (int)(blockIdx.x / kXCDs);
-    if (slot >= a.ntiles) return;   // whole block: uniform
-    const int tile = a.tiles[slot];
-    const int x0 = (tile % a.ntiles_x) * TX, y0 = (tile / a.ntiles_x) * TY;
-    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = a.w[i];
-    for (int i = threadIdx.x; i < plane; i += blockDim.x) {
-        int gx = x0 - r + i % LW, gy = y0 - r + i / LW;
-        bool in = (unsigned)gx < (unsigned)a.tW && (unsigned)gy < (unsigned)a.tH;
-        size_t q = in ? (size_t)gy * a.tW + gx : 0;
-        float *px = lds + (size_t)i * FP;
-#pragma unroll
-        for (int c = 0; c < C; c++) px[c] = in ? a.tgt[q * C + c] : 0.0f;
-#pragma unroll
-        for (int k = 0; k < 2 * C; k++) px[C + k] = in ? a.tgrad[q * 2 * C + k] : 0.0f;
-#pragma unroll
-        for (int k = F; k < FP; k++) px[k] = 0.0f;
-    }
-    __syncthreads();
-    const int lane = (int)threadIdx.x & (kLanes - 1);
-    const int wave = (int)threadIdx.x / 64;
-    const int lw = (int)threadIdx.x & 63;
-    const int lx = lw / kLanes;   // target of this lane's group within the wave
-    const int x = x0 + lx, y = y0 + wave;
-    // no early exit below: the wave-level steps need every lane
-    const bool valid = x < a.tW && y < a.tH && is_target(a, x, y);
-    const size_t p = valid ? (size_t)a.tW * y + x : 0;
-    int2 best = valid ? a.nnf_in[p] : make_int2(-1, -1);
-    float bd = valid ? a.dist_in[p] : INFINITY;
-
-    // phase 1: propagation (as k_pm_sweep)
-    {
-        const int st = a.step;
-        const int ox = lane == 0 ? -st : (lane == 1 ? st : 0);
-        const int oy = lane == 2 ? -st : (lane == 3 ? st : 0);
-        float d = INFINITY;
-        int2 c = best;
-        int order = INT_MAX;
-        const int qx = x + ox, qy = y + oy;
-        if (valid && is_target(a, qx, qy)) {
-            int2 e = a.nnf_in[(size_t)a.tW * qy + qx];
-            e.x -= ox;
-            e.y -= oy;
-            if ((e.x != best.x || e.y != best.y) && is_source(a, e.x, e.y)) {
-                const TgtLdsI<FP> t{lds, LW, lx, wave};
-                d = patch_distance_fly<KIND, C, PS>(a, wl, e.x, e.y, t, bd);
-                c = e;
-                order = lane;
-            }
-        }
-        group_min(d, c, order);
-        if (d < bd) {
-            bd = d;
-            best = c;
-        }
-    }
-    if (lane == 0) {
-        pl.best1[wave * kWaveTargets + lx] = best;
-        pl.bd1[wave * kWaveTargets + lx] = valid ? bd : -1.0f;
-        pl.best[wave * kWaveTargets + lx] = pool_key(INFINITY, 0x7FFFFFFF);
-    }
-    wave_sync();
-
-    // phase 2a: draw the candidate of every (target, window) of the wave
-    int R = 0;
-    for (int ws = a.maxw; ws >= 1; ws /= 2) R++;
-    const int ntask = kWaveTargets * R;   // task j: target j % 16, window j / 16
-    int *cand = pl.cand + wave * kWaveTargets * kMaxR;
-    uint8_t *list = pl.list + wave * kWaveTargets * kMaxR;
-    int nvalid = 0;
-    for (int base = 0; base < ntask; base += 64) {
-        const int j = base + lw;
-        bool ok = false;
-        if (j < ntask) {
-            const int t = j % kWaveTargets, wi = j / kWaveTargets;
-            int packed = -1;
-            if (pl.bd1[wave * kWaveTargets + t] >= 0.0f) {
-                const int2 ctr = pl.best1[wave * kWaveTargets + t];
-                const uint32_t pt = (uint32_t)((size_t)a.tW * (y0 + wave) + x0 + t);
-                Rng rng = make_rng(a.seed, a.stream * 32u + (uint32_t)wi, pt);
-                const int ws = a.maxw >> wi;
-                const int x_min = max(ctr.x - ws, 0), y_min = max(ctr.y - ws, 0);
-                const int x_max = min(ctr.x + ws + 1, a.sW), y_max = min(ctr.y + ws + 1, a.sH);
-                for (int k = 0; k < a.shots; k++) {
-                    const int cx = x_min + rng.below(x_max - x_min);
-                    const int cy = y_min + rng.below(y_max - y_min);
-                    if (is_source(a, cx, cy)) {
-                        // the phase-1 best itself is never strictly better: no evaluation
-                        ok = cx != ctr.x || cy != ctr.y;
-                        if (ok) packed = cx | (cy << 16);
-                        break;
-                    }
-                }
-            }
-            cand[j] = packed;
-        }
-        const uint64_t m = __ballot(ok);
-        if (ok) list[nvalid + lane_rank(m)] = (uint8_t)j;
-        nvalid += __popcll(m);
-    }
-    wave_sync();
-
-    // phase 2b: evaluate the pool, smallest windows first
     const double lam = (double)a.lambda, lam1 = (double)(1 - a.lambda);
     const double lamd = (double)a.lambda_d, lamd1 = (double)(1 - a.lambda_d);
     const size_t stride = (size_t)a.sW * C;
-    uint64_t *bestk = pl.best + wave * kWaveTargets;
-    const float *bd1 = pl.bd1 + wave * kWaveTargets;
     bool active = false;
-    int t = 0, wi = 0, cy = 0, dy = 0, last = P;
+    int t = 0, order = 0, cy = 0, dy = 0, last = P;
     float bound = 0.0f;
     const float *row = a.src;
     double dist = 0.0, gdist = 0.0, dist_d = 0.0, gdist_d = 0.0;
@@ -743,10 +633,6 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
         if constexpr (KIND == DIST_L2RGBD) v = v + lamd * dist_d + lamd1 * gdist_d;
         return v;
     };
-    auto finish = [&]() {
-        const float d = (float)(partial() / (double)PP);
-        if (d <= bound) atomicMin((unsigned long long *)&bestk[t], (unsigned long long)pool_key(d, wi));
-    };
     int next = 0;
     for (;;) {
         const bool need = !active;
@@ -754,13 +640,13 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
         if (need) {
             const int k = next + lane_rank(m);
             if (k < nvalid) {
-                const int j = list[nvalid - 1 - k];
+                const int j = wp.list[LAST_FIRST ? nvalid - 1 - k : k];
                 t = j % kWaveTargets;
-                wi = j / kWaveTargets;
-                const int packed = cand[j];
+                order = j / kWaveTargets;
+                const int packed = wp.cand[j];
                 const int cx = packed & 0xFFFF;
                 cy = packed >> 16;
-                bound = fminf(bd1[t], key_dist(bestk[t]));
+                bound = fminf(wp.bd[t], key_dist(wp.keys[t]));
                 dy = 0;
                 dist = gdist = dist_d = gdist_d = 0.0;
                 last = (cx + r + 1 < a.sW) ? P : P - 1;
@@ -773,15 +659,17 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
         if (__ballot(active) == 0) break;
         if (active) {
             // one patch row per pass: row dy is in A, row dy + 1 goes to B, then B -> A
-            bound = fminf(bound, key_dist(bestk[t]));
-            const TgtLdsI<FP> tg{lds, LW, t, wave};
+            bound = fminf(bound, key_dist(wp.keys[t]));
+            const TgtLdsI<FP> tg{tiles + (t / TX) * tile_floats, LW, t % TX, wave};
             const float *nrow = (cy - r + dy + 1 < a.sH) ? row + stride : row;
             load_row<C, P>(B, nrow, last);
             fly_row<KIND, C, P>(A, B, wl, tg, dy, dist, gdist, dist_d, gdist_d);
             if (partial() > (double)bound * (double)PP * (1.0 + 1e-6)) {
                 active = false;
             } else if (dy + 1 >= P) {
-                finish();
+                const float d = (float)(partial() / (double)PP);
+                if (d <= bound)
+                    atomicMin((unsigned long long *)&wp.keys[t], (unsigned long long)pool_key(d, order));
                 active = false;
             } else {
 #pragma unroll
@@ -793,17 +681,169 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
             }
         }
     }
+}
+
+template <int KIND, int C, int PS>
+__global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
+{
+    static_assert(PS > 0 && (KIND == DIST_L2RGBD || (KIND == DIST_L2COMBINED && (C == 1 || C == 3))),
+                  "pool sweep: gradient distances with a compile-time patch");
+    extern __shared__ __align__(16) float plds[];
+    constexpr int F = 3 * C;
+    constexpr int FP = pool_feat_stride(F);
+    constexpr int P = PS, r = PS / 2;
+    constexpr int LW = TX + 2 * r, LH = TY + 2 * r;
+    constexpr int plane = LW * LH;
+    constexpr int tile_floats = plane * FP;
+    float *tiles = plds;
+    float *wl = plds + kTilesPerBlock * tile_floats;
+    int R = 0;
+    for (int ws = a.maxw; ws >= 1; ws /= 2) R++;
+    const int ntask = pool_tasks_per_target(R);
+    const int wave = (int)threadIdx.x / 64;
+    const int lw = (int)threadIdx.x & 63;
+    WavePool wp;
+    {
+        char *pb = reinterpret_cast<char *>(plds) + pool_tables_offset(F, P) + wave * pool_wave_bytes(R);
+        wp.keys = reinterpret_cast<uint64_t *>(pb);
+        wp.best = reinterpret_cast<int2 *>(pb + kWaveTargets * sizeof(uint64_t));
+        wp.bd = reinterpret_cast<float *>(pb + kWaveTargets * (sizeof(uint64_t) + sizeof(int2)));
+        wp.cand = reinterpret_cast<int *>(pb + kWaveTargets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)));
+        wp.list = reinterpret_cast<uint16_t *>(wp.cand + kWaveTargets * ntask);
+    }
+
+    // block -> pair of tile slots; blocks are dealt to the 8 XCDs round-robin, so each
+    // XCD gets a contiguous band of tiles
+    const int per = (int)(gridDim.x / kXCDs);
+    const int slot = (int)(blockIdx.x % kXCDs) * per + (int)(blockIdx.x / kXCDs);
+    if (slot * kTilesPerBlock >= a.ntiles) return;   // whole block: uniform
+    int tx0[kTilesPerBlock], ty0[kTilesPerBlock];
+    for (int ti = 0; ti < kTilesPerBlock; ti++) {
+        const int s = slot * kTilesPerBlock + ti;
+        const int tile = s < a.ntiles ? a.tiles[s] : -1;
+        tx0[ti] = tile >= 0 ? (tile % a.ntiles_x) * TX : -1;
+        ty0[ti] = tile >= 0 ? (tile / a.ntiles_x) * TY : -1;
+    }
+    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = a.w[i];
+    for (int i = threadIdx.x; i < kTilesPerBlock * plane; i += blockDim.x) {
+        const int ti = i / plane, k = i % plane;
+        const int gx = tx0[ti] - r + k % LW, gy = ty0[ti] - r + k / LW;
+        const bool in = tx0[ti] >= 0 && (unsigned)gx < (unsigned)a.tW && (unsigned)gy < (unsigned)a.tH;
+        const size_t q = in ? (size_t)gy * a.tW + gx : 0;
+        float *px = tiles + (size_t)ti * tile_floats + (size_t)k * FP;
+#pragma unroll
+        for (int c = 0; c < C; c++) px[c] = in ? a.tgt[q * C + c] : 0.0f;
+#pragma unroll
+        for (int k2 = 0; k2 < 2 * C; k2++) px[C + k2] = in ? a.tgrad[q * 2 * C + k2] : 0.0f;
+#pragma unroll
+        for (int k2 = F; k2 < FP; k2++) px[k2] = 0.0f;
+    }
+    __syncthreads();
+
+    // targets of this wave: t = 16 * tile + column, row `wave` of the tile
+    auto target_xy = [&](int t, int &x, int &y) {
+        const int ti = t / TX;
+        x = tx0[ti] + t % TX;
+        y = ty0[ti] + wave;
+        return tx0[ti] >= 0 && x < a.tW && y < a.tH && is_target(a, x, y);
+    };
+    if (lw < kWaveTargets) {
+        int x, y;
+        const bool valid = target_xy(lw, x, y);
+        const size_t p = valid ? (size_t)a.tW * y + x : 0;
+        wp.best[lw] = valid ? a.nnf_in[p] : make_int2(-1, -1);
+        wp.bd[lw] = valid ? a.dist_in[p] : -1.0f;
+        wp.keys[lw] = kNoKey;
+    }
     wave_sync();
-    if (valid && lane == 0) {
-        const uint64_t k = bestk[lx];
-        const float gd = key_dist(k);
-        if (gd < bd) {
-            const int packed = cand[(int)(uint32_t)k * kWaveTargets + lx];
-            best = make_int2(packed & 0xFFFF, packed >> 16);
-            bd = gd;
+
+    // phase 1: propagation candidates, task order k * 32 + t (k: left, right, up, down)
+    int nvalid = 0;
+    for (int base = 0; base < 4 * kWaveTargets; base += 64) {
+        const int j = base + lw;
+        const int t = j % kWaveTargets, k = j / kWaveTargets;
+        bool ok = false;
+        int x, y;
+        if (wp.bd[t] >= 0.0f && target_xy(t, x, y)) {
+            const int st = a.step;
+            const int ox = k == 0 ? -st : (k == 1 ? st : 0);
+            const int oy = k == 2 ? -st : (k == 3 ? st : 0);
+            if (is_target(a, x + ox, y + oy)) {
+                int2 e = a.nnf_in[(size_t)a.tW * (y + oy) + x + ox];
+                e.x -= ox;
+                e.y -= oy;
+                const int2 cur = wp.best[t];
+                if ((e.x != cur.x || e.y != cur.y) && is_source(a, e.x, e.y)) {
+                    wp.cand[j] = e.x | (e.y << 16);
+                    ok = true;
+                }
+            }
         }
-        a.nnf_out[p] = best;
-        a.dist_out[p] = bd;
+        const uint64_t m = __ballot(ok);
+        if (ok) wp.list[nvalid + lane_rank(m)] = (uint16_t)j;
+        nvalid += __popcll(m);
+    }
+    wave_sync();
+    run_pool<KIND, C, P, FP, false>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);
+    wave_sync();
+    if (lw < kWaveTargets) {
+        const uint64_t k = wp.keys[lw];
+        if (wp.bd[lw] >= 0.0f && key_dist(k) < wp.bd[lw]) {
+            const int packed = wp.cand[(int)(uint32_t)k * kWaveTargets + lw];
+            wp.best[lw] = make_int2(packed & 0xFFFF, packed >> 16);
+            wp.bd[lw] = key_dist(k);
+        }
+        wp.keys[lw] = kNoKey;
+    }
+    wave_sync();
+
+    // phase 2: random search around the phase-1 best, task order wi * 32 + t
+    nvalid = 0;
+    for (int base = 0; base < R * kWaveTargets; base += 64) {
+        const int j = base + lw;
+        bool ok = false;
+        if (j < R * kWaveTargets) {
+            const int t = j % kWaveTargets, wi = j / kWaveTargets;
+            int x, y;
+            if (wp.bd[t] >= 0.0f && target_xy(t, x, y)) {
+                const int2 ctr = wp.best[t];
+                Rng rng = make_rng(a.seed, a.stream * 32u + (uint32_t)wi, (uint32_t)((size_t)a.tW * y + x));
+                const int ws = a.maxw >> wi;
+                const int x_min = max(ctr.x - ws, 0), y_min = max(ctr.y - ws, 0);
+                const int x_max = min(ctr.x + ws + 1, a.sW), y_max = min(ctr.y + ws + 1, a.sH);
+                for (int k = 0; k < a.shots; k++) {
+                    const int cx = x_min + rng.below(x_max - x_min);
+                    const int cy = y_min + rng.below(y_max - y_min);
+                    if (is_source(a, cx, cy)) {
+                        ok = cx != ctr.x || cy != ctr.y;
+                        if (ok) wp.cand[j] = cx | (cy << 16);
+                        break;
+                    }
+                }
+            }
+        }
+        const uint64_t m = __ballot(ok);
+        if (ok) wp.list[nvalid + lane_rank(m)] = (uint16_t)j;
+        nvalid += __popcll(m);
+    }
+    wave_sync();
+    run_pool<KIND, C, P, FP, true>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);   // smallest windows first
+    wave_sync();
+    if (lw < kWaveTargets) {
+        int x, y;
+        if (target_xy(lw, x, y)) {
+            const uint64_t k = wp.keys[lw];
+            int2 best = wp.best[lw];
+            float bd = wp.bd[lw];
+            if (key_dist(k) < bd) {
+                const int packed = wp.cand[(int)(uint32_t)k * kWaveTargets + lw];
+                best = make_int2(packed & 0xFFFF, packed >> 16);
+                bd = key_dist(k);
+            }
+            const size_t p = (size_t)a.tW * y + x;
+            a.nnf_out[p] = best;
+            a.dist_out[p] = bd;
+        }
     }
 }
 
@@ -1045,7 +1085,8 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     const bool pool = R <= kMaxR && ((d.kind == DIST_L2COMBINED && C == 3 &&
                                       (d.patch == 7 || d.patch == 9 || d.patch == 11)) ||
                                      (d.kind == DIST_L2RGBD && C == 4 && d.patch == 9));
-    const size_t lds_pool = pool_lds_offset(F, d.patch) + pool_lds_bytes();
+    const size_t lds_pool = pool_tables_offset(F, d.patch) + kWaves * pool_wave_bytes(R);
+    const int pool_blocks = ((pr.ntiles + kTilesPerBlock - 1) / kTilesPerBlock + kXCDs - 1) / kXCDs * kXCDs;
     for (int k = 0; k < K; k++) {
         a.nnf_in = n_cur;
         a.dist_in = d_cur;
@@ -1056,13 +1097,13 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
         prof::begin(TK_PM_SWEEP, s);
         if (pool) {
             if (d.kind == DIST_L2RGBD)
-                k_pm_sweep_pool<DIST_L2RGBD, 4, 9><<<sweep_blocks, kBlock, lds_pool, s>>>(a);
+                k_pm_sweep_pool<DIST_L2RGBD, 4, 9><<<pool_blocks, kBlock, lds_pool, s>>>(a);
             else if (d.patch == 7)
-                k_pm_sweep_pool<DIST_L2COMBINED, 3, 7><<<sweep_blocks, kBlock, lds_pool, s>>>(a);
+                k_pm_sweep_pool<DIST_L2COMBINED, 3, 7><<<pool_blocks, kBlock, lds_pool, s>>>(a);
             else if (d.patch == 9)
-                k_pm_sweep_pool<DIST_L2COMBINED, 3, 9><<<sweep_blocks, kBlock, lds_pool, s>>>(a);
+                k_pm_sweep_pool<DIST_L2COMBINED, 3, 9><<<pool_blocks, kBlock, lds_pool, s>>>(a);
             else
-                k_pm_sweep_pool<DIST_L2COMBINED, 3, 11><<<sweep_blocks, kBlock, lds_pool, s>>>(a);
+                k_pm_sweep_pool<DIST_L2COMBINED, 3, 11><<<pool_blocks, kBlock, lds_pool, s>>>(a);
         } else {
 #define NLI_SWEEP(KD, CC, PP) k_pm_sweep<KD, CC, PP><<<sweep_blocks, kBlock, lds, s>>>(a)
             NLI_DISPATCH(d.kind, C, d.patch, NLI_SWEEP);
