@@ -93,7 +93,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--images", type=int, default=8, help="1024x1024 images per step")
-    ap.add_argument("--workers", type=int, default=4, help="concurrent per-image drivers")
+    ap.add_argument("--workers", type=int, default=8, help="concurrent per-image drivers")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the single-image and FP64 runs")

@@ -239,18 +239,15 @@ __global__ void k_write_back(float *__restrict__ img, int Cs, int c0, int nc, co
 
 // ---------------- NL-Poisson coefficients ----------------
 // a1, a2: W*H; f1, F1, F2: nc planes of W*H (double)
-__global__ void k_poisson_coef(const float *__restrict__ img, const float *__restrict__ grad, int Cs, int c0,
-                               int nc, const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
-                               const float *__restrict__ conf, const float *__restrict__ w, int P, float lambda,
-                               int W, int H, double *__restrict__ a1, double *__restrict__ a2,
-                               double *__restrict__ f1, double *__restrict__ F1, double *__restrict__ F2)
+// Coefficients at pixel q: gather from the centres p around q (row-major, the order in
+// which the reference's scatter reaches q).  The source gradient is formed from the
+// image with gradient.cpp's float forward differences (0 past the last column / row).
+__device__ inline void poisson_coef_at(int qx, int qy, const float *__restrict__ img, int Cs, int c0, int nc,
+                                       const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
+                                       const float *__restrict__ conf, const float *__restrict__ wl, int P,
+                                       float lambda, int W, int H, double *__restrict__ a1, double *__restrict__ a2,
+                                       double *__restrict__ f1, double *__restrict__ F1, double *__restrict__ F2)
 {
-    __shared__ float wl[kMaxP * kMaxP];
-    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
-    __syncthreads();
-    int qx = blockIdx.x * blockDim.x + threadIdx.x;
-    int qy = blockIdx.y;
-    if (qx >= W) return;
     const int h = P / 2;
     const size_t plane = (size_t)W * H;
     const size_t q = (size_t)W * qy + qx;
@@ -271,12 +268,13 @@ __global__ void k_poisson_coef(const float *__restrict__ img, const float *__res
             A2 += weight * lam1;
             size_t nidx = (size_t)W * n_y + n_x;
             const float *iv = img + nidx * Cs + c0;
-            const float *gv = grad + (nidx * Cs + c0) * 2;
             for (int ch = 0; ch < kMaxNC; ch++)
                 if (ch < nc) {
+                    const float gx = n_x + 1 < W ? iv[ch + Cs] - iv[ch] : 0.0f;
+                    const float gy = n_y + 1 < H ? iv[ch + (size_t)W * Cs] - iv[ch] : 0.0f;
                     v1[ch] += weight * (double)iv[ch] * lam;
-                    g1[ch] += weight * (double)gv[2 * ch] * lam1;
-                    g2[ch] += weight * (double)gv[2 * ch + 1] * lam1;
+                    g1[ch] += weight * (double)gx * lam1;
+                    g2[ch] += weight * (double)gy * lam1;
                 }
         }
     a1[q] = A1;
@@ -288,20 +286,46 @@ __global__ void k_poisson_coef(const float *__restrict__ img, const float *__res
     }
 }
 
-// backward divergence of (F1, F2) at extended-domain points (patch_non_local_poisson.cpp:235-285)
-__global__ void k_divergence(const uint8_t *__restrict__ ext, const double *__restrict__ F1,
+// Coefficients where the solve reads them: every hole point, and its left and top
+// neighbours (the stencil's a2 and the divergence's F there).  A neighbour that is itself
+// a hole point is left to its own thread; one that is the neighbour of two hole points
+// is written twice with the same values.
+__global__ void k_poisson_coef(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
+                               const float *__restrict__ img, int Cs, int c0, int nc,
+                               const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
+                               const float *__restrict__ conf, const float *__restrict__ w, int P, float lambda,
+                               int W, int H, double *__restrict__ a1, double *__restrict__ a2,
+                               double *__restrict__ f1, double *__restrict__ F1, double *__restrict__ F2)
+{
+    __shared__ float wl[kMaxP * kMaxP];
+    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
+    __syncthreads();
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int p = pts[k];
+    const int x = p % W, y = p / W;
+    poisson_coef_at(x, y, img, Cs, c0, nc, ext, nnf, conf, wl, P, lambda, W, H, a1, a2, f1, F1, F2);
+    if (x > 0 && lidx[p - 1] < 0)
+        poisson_coef_at(x - 1, y, img, Cs, c0, nc, ext, nnf, conf, wl, P, lambda, W, H, a1, a2, f1, F1, F2);
+    if (y > 0 && lidx[p - W] < 0)
+        poisson_coef_at(x, y - 1, img, Cs, c0, nc, ext, nnf, conf, wl, P, lambda, W, H, a1, a2, f1, F1, F2);
+}
+
+// backward divergence of (F1, F2) (patch_non_local_poisson.cpp:235-285, with its
+// boundary cases) at the hole points, the only points the right-hand side reads it at
+__global__ void k_divergence(const int *__restrict__ pts, int n, const double *__restrict__ F1,
                              const double *__restrict__ F2, int nc, int W, int H, int active,
                              double *__restrict__ f2)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    if (x >= W) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
     const size_t plane = (size_t)W * H;
-    const size_t idx = (size_t)W * y + x;
+    const size_t idx = (size_t)pts[k];
+    const int x = (int)(idx % W), y = (int)(idx / W);
     for (int ch = 0; ch < nc; ch++) {
         const double *fx = F1 + ch * plane, *fy = F2 + ch * plane;
         double dv = 0.0;
-        if (active && ext[idx]) {
+        if (active) {
             if (x == 0) dv = fx[idx];
             else if (x == W - 1) dv = -fx[idx - 1];
             else dv = fx[idx] - fx[idx - 1];
@@ -770,22 +794,12 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
 {
     const size_t np = (size_t)W * H;
     upload_weights(w, patch, sigma, s);
-    w.grad.ensure(np * Cs * 2);
-    gradient(image, W, H, Cs, w.grad.p, s);
     w.a1.ensure(np);
     w.a2.ensure(np);
     w.f1.ensure(np * nc);
     w.F1.ensure(np * nc);
     w.F2.ensure(np * nc);
     w.f2.ensure(np * nc);
-    dim3 g2((W + kBlock - 1) / kBlock, H);
-    prof::begin(TK_POISSON_COEF, s);
-    k_poisson_coef<<<g2, kBlock, 0, s>>>(image, w.grad.p, Cs, c0, nc, ext, (const int2 *)nnf, conf,
-                                         w.weights.p, patch, lambda, W, H, w.a1.p, w.a2.p, w.f1.p, w.F1.p, w.F2.p);
-    NLI_LAUNCH_CHECK();
-    prof::end(TK_POISSON_COEF, s, 0.0, (double)np);
-    k_divergence<<<g2, kBlock, 0, s>>>(ext, w.F1.p, w.F2.p, nc, W, H, lambda != 1 ? 1 : 0, w.f2.p);
-    NLI_LAUNCH_CHECK();
     const int nb = (int)grid_for(std::max(n, 1));
     w.scalars.ensure(kMaxNC * S_COUNT + 8 + (size_t)kMaxNC * (cg.limit + 2));
     w.partials.ensure((size_t)2 * kMaxNC * nb + 8);
@@ -801,6 +815,14 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     w.lidx.ensure(np);
     fill_i32(w.lidx.p, -1, np, s);
     k_lidx<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p);
+    NLI_LAUNCH_CHECK();
+    prof::begin(TK_POISSON_COEF, s);
+    k_poisson_coef<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p, image, Cs, c0, nc, ext, (const int2 *)nnf,
+                                                  conf, w.weights.p, patch, lambda, W, H, w.a1.p, w.a2.p, w.f1.p,
+                                                  w.F1.p, w.F2.p);
+    NLI_LAUNCH_CHECK();
+    prof::end(TK_POISSON_COEF, s, 0.0, (double)n);
+    k_divergence<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.F1.p, w.F2.p, nc, W, H, lambda != 1 ? 1 : 0, w.f2.p);
     NLI_LAUNCH_CHECK();
     w.stencil.ensure((size_t)n * (sizeof(Stencil) / sizeof(double)));
     w.nbr.ensure((size_t)n * 4);
