@@ -305,6 +305,17 @@ int nli_inpaint_batch_dev(nli_ctx *ctx, const nli_inpaint_params *params, int n,
                           const float *const *in, const uint8_t *const *mask, const int *W,
                           const int *H, int C, float *const *out, int workers,
                           nli_inpaint_stats *stats);
+/* as nli_inpaint_batch(_dev) with one parameter record per image (params[n]): methods,
+ * patch sides, scales and seeds may differ between images.  Images are run largest
+ * first; results do not depend on the order. */
+int nli_inpaint_batch_each(nli_ctx *ctx, const nli_inpaint_params *params, int n,
+                           const float *const *in, const uint8_t *const *mask, const int *W,
+                           const int *H, int C, float *const *out, int workers,
+                           nli_inpaint_stats *stats);
+int nli_inpaint_batch_each_dev(nli_ctx *ctx, const nli_inpaint_params *params, int n,
+                               const float *const *in, const uint8_t *const *mask, const int *W,
+                               const int *H, int C, float *const *out, int workers,
+                               nli_inpaint_stats *stats);
 
 /* fills params with the reference CLI defaults (main.cpp:68-82, 220-233, 254, 266) */
 void nli_inpaint_params_default(nli_inpaint_params *p);

@@ -43,7 +43,8 @@ EXPORTS = [
     "nli_patchmatch", "nli_patchmatch_dev", "nli_update", "nli_update_dev", "nli_poisson_fill",
     "nli_inpaint", "nli_inpaint_dev", "nli_inpaint_params_default", "nli_profile_enable",
     "nli_patch_energy", "nli_energy_gradient", "nli_patch_extract", "nli_adjoint_aggregate",
-    "nli_profile_read", "nli_inpaint_batch", "nli_inpaint_batch_dev",
+    "nli_profile_read", "nli_inpaint_batch", "nli_inpaint_batch_dev", "nli_inpaint_batch_each",
+    "nli_inpaint_batch_each_dev",
 ]
 # symbols of libnli_plugins.so (include/nli_plugins.hpp)
 PLUGIN_EXPORTS = ["nli_plain_inpaint", "nli_plain_last_error"]
@@ -484,8 +485,9 @@ class Context:
         return stats_of(st)
 
     # ---- batches ----
-    def inpaint_batch(self, images, masks, params: "Inpainting", workers: int = 0):
-        """Inpaints a list of images (uneven sizes allowed) concurrently; returns
+    def inpaint_batch(self, images, masks, params, workers: int = 0):
+        """Inpaints a list of images (uneven sizes allowed) concurrently; params is one
+        Inpainting for all images or a list with one per image.  Returns
         (outputs, [RunStats])."""
         images = [_c(im, np.float32) for im in images]
         masks = [_c(m, np.uint8) for m in masks]
@@ -500,19 +502,30 @@ class Context:
         Ws = (ct.c_int * n)(*[im.shape[1] for im in images])
         Hs = (ct.c_int * n)(*[im.shape[0] for im in images])
         st = (InpaintStats * n)()
-        p = params.to_struct()
-        self._check(self.lib.nli_inpaint_batch(self.h, ct.byref(p), n, ins, ms, Ws, Hs, C, os_, workers, st))
+        if isinstance(params, (list, tuple)):
+            assert len(params) == n
+            ps = (InpaintParams * n)(*[q.to_struct() for q in params])
+            self._check(self.lib.nli_inpaint_batch_each(self.h, ps, n, ins, ms, Ws, Hs, C, os_, workers, st))
+        else:
+            p = params.to_struct()
+            self._check(self.lib.nli_inpaint_batch(self.h, ct.byref(p), n, ins, ms, Ws, Hs, C, os_, workers, st))
         return outs, [stats_of(st[i]) for i in range(n)]
 
-    def inpaint_batch_dev(self, d_ins, d_masks, d_outs, Ws, Hs, C, params: "Inpainting", workers: int = 0):
-        """Device-resident batch: lists of device pointers (from to_device / dev_alloc)."""
+    def inpaint_batch_dev(self, d_ins, d_masks, d_outs, Ws, Hs, C, params, workers: int = 0):
+        """Device-resident batch: lists of device pointers (from to_device / dev_alloc);
+        params as in inpaint_batch."""
         n = len(d_ins)
         V = ct.c_void_p
         st = (InpaintStats * n)()
-        p = params.to_struct()
-        self._check(self.lib.nli_inpaint_batch_dev(self.h, ct.byref(p), n, (V * n)(*d_ins), (V * n)(*d_masks),
-                                                   (ct.c_int * n)(*Ws), (ct.c_int * n)(*Hs), C, (V * n)(*d_outs),
-                                                   workers, st))
+        args = ((V * n)(*d_ins), (V * n)(*d_masks), (ct.c_int * n)(*Ws), (ct.c_int * n)(*Hs), C, (V * n)(*d_outs),
+                workers, st)
+        if isinstance(params, (list, tuple)):
+            assert len(params) == n
+            ps = (InpaintParams * n)(*[q.to_struct() for q in params])
+            self._check(self.lib.nli_inpaint_batch_each_dev(self.h, ps, n, *args))
+        else:
+            p = params.to_struct()
+            self._check(self.lib.nli_inpaint_batch_dev(self.h, ct.byref(p), n, *args))
         return [stats_of(st[i]) for i in range(n)]
 
     # ---- full process ----

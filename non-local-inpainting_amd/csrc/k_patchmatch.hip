@@ -617,6 +617,7 @@ template <int KIND, int C, int P, int FP, bool LAST_FIRST>
 __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const float *tiles, int LW, int tile_floats,
                          int wave, const WavePool &wp, int nvalid)
 {
+    constexpr bool GRAD = Feat<KIND>::grad;
     constexpr int r = P / 2;
     constexpr unsigned PP = (unsigned)(P * P);
     const double lam = (double)a.lambda, lam1 = (double)(1 - a.lambda);
@@ -626,12 +627,16 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
     int t = 0, order = 0, cy = 0, dy = 0, last = P;
     float bound = 0.0f;
     const float *row = a.src;
-    double dist = 0.0, gdist = 0.0, dist_d = 0.0, gdist_d = 0.0;
-    float A[P + 1][C], B[P + 1][C];
+    double dist = 0.0, gdist = 0.0, dist_d = 0.0, gdist_d = 0.0;   // gradient distances (FP64)
+    float facc = 0.0f;                                             // L2 / L1 norms (FP32)
+    float A[GRAD ? P + 1 : 1][C];
     auto partial = [&]() {
         double v = lam * dist + lam1 * gdist;
         if constexpr (KIND == DIST_L2RGBD) v = v + lamd * dist_d + lamd1 * gdist_d;
         return v;
+    };
+    auto record = [&](float d) {
+        if (d <= bound) atomicMin((unsigned long long *)&wp.keys[t], (unsigned long long)pool_key(d, order));
     };
     int next = 0;
     for (;;) {
@@ -648,36 +653,64 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
                 cy = packed >> 16;
                 bound = fminf(wp.bd[t], key_dist(wp.keys[t]));
                 dy = 0;
-                dist = gdist = dist_d = gdist_d = 0.0;
-                last = (cx + r + 1 < a.sW) ? P : P - 1;
                 row = a.src + ((size_t)(cy - r) * a.sW + (cx - r)) * C;
-                load_row<C, P>(A, row, last);
+                if constexpr (GRAD) {
+                    dist = gdist = dist_d = gdist_d = 0.0;
+                    last = (cx + r + 1 < a.sW) ? P : P - 1;
+                    load_row<C, P>(A, row, last);
+                } else {
+                    facc = 0.0f;
+                }
                 active = true;
             }
         }
         next += __popcll(m);
         if (__ballot(active) == 0) break;
         if (active) {
-            // one patch row per pass: row dy is in A, row dy + 1 goes to B, then B -> A
             bound = fminf(bound, key_dist(wp.keys[t]));
             const TgtLdsI<FP> tg{tiles + (t / TX) * tile_floats, LW, t % TX, wave};
-            const float *nrow = (cy - r + dy + 1 < a.sH) ? row + stride : row;
-            load_row<C, P>(B, nrow, last);
-            fly_row<KIND, C, P>(A, B, wl, tg, dy, dist, gdist, dist_d, gdist_d);
-            if (partial() > (double)bound * (double)PP * (1.0 + 1e-6)) {
-                active = false;
-            } else if (dy + 1 >= P) {
-                const float d = (float)(partial() / (double)PP);
-                if (d <= bound)
-                    atomicMin((unsigned long long *)&wp.keys[t], (unsigned long long)pool_key(d, order));
-                active = false;
+            if constexpr (GRAD) {
+                // one patch row per pass: row dy is in A, row dy + 1 goes to B, then B -> A
+                float B[P + 1][C];
+                const float *nrow = (cy - r + dy + 1 < a.sH) ? row + stride : row;
+                load_row<C, P>(B, nrow, last);
+                fly_row<KIND, C, P>(A, B, wl, tg, dy, dist, gdist, dist_d, gdist_d);
+                if (partial() > (double)bound * (double)PP * (1.0 + 1e-6)) {
+                    active = false;
+                } else if (dy + 1 >= P) {
+                    record((float)(partial() / (double)PP));
+                    active = false;
+                } else {
+#pragma unroll
+                    for (int i = 0; i <= P; i++)
+#pragma unroll
+                        for (int c = 0; c < C; c++) A[i][c] = B[i][c];
+                    row = nrow;
+                    dy++;
+                }
             } else {
+                // colour-only distances: float accumulation in row-major order, as
+                // l2_norm_patch_distance.cpp / l1_norm_patch_distance.cpp
 #pragma unroll
-                for (int i = 0; i <= P; i++)
+                for (int dx = 0; dx < P; dx++) {
+                    float norm = 0.0f;
 #pragma unroll
-                    for (int c = 0; c < C; c++) A[i][c] = B[i][c];
-                row = nrow;
-                dy++;
+                    for (int c = 0; c < C; c++) {
+                        float d = row[dx * C + c] - tg.v(c, dx, dy);
+                        float term = (KIND == DIST_L1) ? fabsf(d) : d * d;
+                        norm = (c == 0) ? term : norm + term;
+                    }
+                    facc += wl[dy * P + dx] * norm;
+                }
+                if (facc > bound * (float)PP * (1.0f + 0x1p-20f)) {
+                    active = false;
+                } else if (dy + 1 >= P) {
+                    record(facc / (float)PP);
+                    active = false;
+                } else {
+                    row += stride;
+                    dy++;
+                }
             }
         }
     }
@@ -686,10 +719,12 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
 template <int KIND, int C, int PS>
 __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
 {
-    static_assert(PS > 0 && (KIND == DIST_L2RGBD || (KIND == DIST_L2COMBINED && (C == 1 || C == 3))),
-                  "pool sweep: gradient distances with a compile-time patch");
+    static_assert(PS > 0 && (KIND == DIST_L2RGBD || KIND == DIST_L2 || KIND == DIST_L1 ||
+                             (KIND == DIST_L2COMBINED && (C == 1 || C == 3))),
+                  "pool sweep: compile-time patch; gradient distances need the on-the-fly source gradient");
     extern __shared__ __align__(16) float plds[];
-    constexpr int F = 3 * C;
+    constexpr bool GRAD = Feat<KIND>::grad;
+    constexpr int F = GRAD ? 3 * C : C;
     constexpr int FP = pool_feat_stride(F);
     constexpr int P = PS, r = PS / 2;
     constexpr int LW = TX + 2 * r, LH = TY + 2 * r;
@@ -733,8 +768,10 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
         float *px = tiles + (size_t)ti * tile_floats + (size_t)k * FP;
 #pragma unroll
         for (int c = 0; c < C; c++) px[c] = in ? a.tgt[q * C + c] : 0.0f;
+        if constexpr (GRAD) {
 #pragma unroll
-        for (int k2 = 0; k2 < 2 * C; k2++) px[C + k2] = in ? a.tgrad[q * 2 * C + k2] : 0.0f;
+            for (int k2 = 0; k2 < 2 * C; k2++) px[C + k2] = in ? a.tgrad[q * 2 * C + k2] : 0.0f;
+        }
 #pragma unroll
         for (int k2 = F; k2 < FP; k2++) px[k2] = 0.0f;
     }
@@ -1084,7 +1121,9 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     // candidate-pool sweep for the specialised gradient distances (up to kMaxR windows)
     const bool pool = R <= kMaxR && ((d.kind == DIST_L2COMBINED && C == 3 &&
                                       (d.patch == 7 || d.patch == 9 || d.patch == 11)) ||
-                                     (d.kind == DIST_L2RGBD && C == 4 && d.patch == 9));
+                                     (d.kind == DIST_L2RGBD && C == 4 && d.patch == 9) ||
+                                     ((d.kind == DIST_L2 || d.kind == DIST_L1) && C == 3 &&
+                                      (d.patch == 7 || d.patch == 9)));
     const size_t lds_pool = pool_tables_offset(F, d.patch) + kWaves * pool_wave_bytes(R);
     const int pool_blocks = ((pr.ntiles + kTilesPerBlock - 1) / kTilesPerBlock + kXCDs - 1) / kXCDs * kXCDs;
     for (int k = 0; k < K; k++) {
@@ -1098,6 +1137,12 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
         if (pool) {
             if (d.kind == DIST_L2RGBD)
                 k_pm_sweep_pool<DIST_L2RGBD, 4, 9><<<pool_blocks, kBlock, lds_pool, s>>>(a);
+            else if (d.kind == DIST_L2)
+                (d.patch == 7 ? k_pm_sweep_pool<DIST_L2, 3, 7> : k_pm_sweep_pool<DIST_L2, 3, 9>)
+                    <<<pool_blocks, kBlock, lds_pool, s>>>(a);
+            else if (d.kind == DIST_L1)
+                (d.patch == 7 ? k_pm_sweep_pool<DIST_L1, 3, 7> : k_pm_sweep_pool<DIST_L1, 3, 9>)
+                    <<<pool_blocks, kBlock, lds_pool, s>>>(a);
             else if (d.patch == 7)
                 k_pm_sweep_pool<DIST_L2COMBINED, 3, 7><<<pool_blocks, kBlock, lds_pool, s>>>(a);
             else if (d.patch == 9)

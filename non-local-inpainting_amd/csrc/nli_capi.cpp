@@ -7,6 +7,8 @@
 #include "nli.h"
 #include "nli_internal.h"
 
+#include <algorithm>
+
 #include <atomic>
 #include <cfloat>
 #include <cstring>
@@ -241,15 +243,25 @@ namespace {
 
 // Runs the batch on `workers` host threads, each driving its own stream; images are
 // taken from a shared counter.  staged = false: in/mask/out are device pointers.
-void run_batch(nli_ctx *ctx, const nli_inpaint_params *params, int n, const float *const *in,
+// params: one record for all images (per_image = false) or one per image
+void run_batch(nli_ctx *ctx, const nli_inpaint_params *params, bool per_image, int n, const float *const *in,
                const uint8_t *const *mask, const int *W, const int *H, int C, float *const *out, int workers,
                nli_inpaint_stats *stats, bool staged)
 {
-    nli::InpaintConfig cfg = inpaint_config(params);
-    need(n >= 0 && in && mask && W && H && out, "bad batch arguments");
+    need(n >= 0 && in && mask && W && H && out && params, "bad batch arguments");
     need(C >= 1 && C <= 4, "image must have 1 to 4 channels");
     for (int i = 0; i < n; i++) need(in[i] && mask[i] && out[i] && W[i] > 0 && H[i] > 0, "bad batch image");
     if (n == 0) return;
+    std::vector<nli::InpaintConfig> cfgs;
+    for (int i = 0; i < (per_image ? n : 1); i++) cfgs.push_back(inpaint_config(params + i));
+    // largest images first (longest-processing-time order): the batch ends sooner, and
+    // each image's result does not depend on when or where it runs
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const nli::InpaintConfig &ca = cfgs[per_image ? a : 0], &cb = cfgs[per_image ? b : 0];
+        return (double)W[a] * H[a] * ca.scales * ca.patch > (double)W[b] * H[b] * cb.scales * cb.patch;
+    });
     if (workers <= 0) workers = 4;
     workers = std::min(workers, n);
     while ((int)ctx->workers.size() < workers) {
@@ -264,7 +276,9 @@ void run_batch(nli_ctx *ctx, const nli_inpaint_params *params, int n, const floa
     auto body = [&](nli_worker *wk) {
         try {
             NLI_HIP(hipSetDevice(ctx->device));
-            for (int i = next++; i < n; i = next++) {
+            for (int k = next++; k < n; k = next++) {
+                const int i = order[k];
+                const nli::InpaintConfig &cfg = cfgs[per_image ? i : 0];
                 const size_t np = (size_t)W[i] * H[i];
                 const float *d_in = in[i];
                 const uint8_t *d_m = mask[i];
@@ -821,14 +835,28 @@ int nli_inpaint_batch(nli_ctx *ctx, const nli_inpaint_params *params, int n, con
                       const uint8_t *const *mask, const int *W, const int *H, int C, float *const *out, int workers,
                       nli_inpaint_stats *stats)
 {
-    return guard(ctx, [&] { run_batch(ctx, params, n, in, mask, W, H, C, out, workers, stats, true); });
+    return guard(ctx, [&] { run_batch(ctx, params, false, n, in, mask, W, H, C, out, workers, stats, true); });
+}
+
+int nli_inpaint_batch_each(nli_ctx *ctx, const nli_inpaint_params *params, int n, const float *const *in,
+                           const uint8_t *const *mask, const int *W, const int *H, int C, float *const *out,
+                           int workers, nli_inpaint_stats *stats)
+{
+    return guard(ctx, [&] { run_batch(ctx, params, true, n, in, mask, W, H, C, out, workers, stats, true); });
+}
+
+int nli_inpaint_batch_each_dev(nli_ctx *ctx, const nli_inpaint_params *params, int n, const float *const *in,
+                               const uint8_t *const *mask, const int *W, const int *H, int C, float *const *out,
+                               int workers, nli_inpaint_stats *stats)
+{
+    return guard(ctx, [&] { run_batch(ctx, params, true, n, in, mask, W, H, C, out, workers, stats, false); });
 }
 
 int nli_inpaint_batch_dev(nli_ctx *ctx, const nli_inpaint_params *params, int n, const float *const *in,
                           const uint8_t *const *mask, const int *W, const int *H, int C, float *const *out,
                           int workers, nli_inpaint_stats *stats)
 {
-    return guard(ctx, [&] { run_batch(ctx, params, n, in, mask, W, H, C, out, workers, stats, false); });
+    return guard(ctx, [&] { run_batch(ctx, params, false, n, in, mask, W, H, C, out, workers, stats, false); });
 }
 
 int nli_inpaint(nli_ctx *ctx, const nli_inpaint_params *params, const float *in, const uint8_t *mask, int W, int H,

@@ -178,3 +178,28 @@ def scaled(w: Workload, width: int, height: int, seed: int = 0) -> Workload:
     else:
         mask = ellipse_holes(width, height, 6, 0.15, seed)
     return Workload(w.name + f"_{width}x{height}", img, mask, img.copy(), dict(w.params))
+
+
+def config_uneven(count: int = 64, seed: int = 3, min_side: int = 128, max_side: int = 1024) -> list:
+    """Uneven batch: ``count`` RGB images with sides drawn uniformly from
+    [min_side, max_side] (independent width and height), random-ellipse holes of
+    5-30 % area, alternately nlmeans and nlmedians, patch 7 or 9, 8 scales.  The
+    coarsest size ratio is 0.1 unless that would leave the coarsest level shorter
+    than 48 px, where no complete source patch survives around a hole (the
+    reference stops with "Empty source mask" there); then it is 48 / min(W, H).
+    Returns a list of Workload."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        W = int(rng.integers(min_side, max_side + 1))
+        H = int(rng.integers(min_side, max_side + 1))
+        frac = float(rng.uniform(0.05, 0.30))
+        img = texture(W, H, 3, 5, seed * 1000 + k)
+        mask = ellipse_holes(W, H, int(rng.integers(2, 6)), frac, seed * 1000 + k)
+        method = "nlmeans" if k % 2 == 0 else "nlmedians"
+        patch = 7 if rng.uniform() < 0.5 else 9
+        coarse = max(0.1, 48.0 / min(W, H))
+        out.append(Workload(f"uneven{k}_{W}x{H}_{method}_p{patch}", img, mask, img.copy(), dict(
+            method=method, patch=patch, sigma=10000.0, scales=8, coarse=coarse, iters=200, init="poisson",
+            seed=k + 1)))
+    return out

@@ -348,6 +348,26 @@ def test_batch_equals_single_images(ctx, syn, nli):
         assert st.iterations == st1.iterations
 
 
+def test_uneven_mixed_batch_equals_single_images(ctx, syn, nli):
+    """Per-image parameters (the uneven batch: mixed sizes, nlmeans / nlmedians, patch
+    7 or 9): each image equals its own single run, whatever the worker and order."""
+    ws = syn.config_uneven(count=8, seed=5, min_side=72, max_side=200)
+    params = []
+    for w in ws:
+        q = dict(w.params)
+        q["iters"] = 6
+        q["scales"] = 4
+        q["coarse"] = max(0.35, 40.0 / min(w.image.shape[:2]))
+        params.append(nli.Inpainting(**q))
+    outs, stats = ctx.inpaint_batch([w.image for w in ws], [w.mask for w in ws], params, workers=3)
+    for w, q, o, st in zip(ws, params, outs, stats):
+        single, st1 = ctx.inpaint(w.image, w.mask, q)
+        assert np.array_equal(o, single), w.name
+        assert st.iterations == st1.iterations
+        sel = w.mask.astype(bool)
+        assert np.array_equal(o[~sel], w.image[~sel])
+
+
 def test_inpaint_errors(ctx, nli):
     img = np.full((40, 40, 3), 0.5, np.float32)
     with pytest.raises(ValueError, match="Mask should not be empty"):
