@@ -132,18 +132,34 @@ void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask,
         NLI_HIP(hipStreamSynchronize(s));
     }
 
+    // PatchMatch and update spans are timed with events on the stream (no host waits);
+    // spans[k] = (start event, end event, is PatchMatch)
+    struct Span {
+        int a, b;
+        bool pm;
+    };
+    std::vector<Span> spans;
+    size_t nev = 0;
+    auto event = [&]() {
+        if (nev == dw.events.size()) {
+            hipEvent_t e;
+            NLI_HIP(hipEventCreate(&e));
+            dw.events.push_back(e);
+        }
+        NLI_HIP(hipEventRecord(dw.events[nev], s));
+        return (int)nev++;
+    };
     uint64_t pm_calls = 0;
     auto run_pm = [&](const float *img, const uint8_t *sm, const uint8_t *tm, int w, int h, const int32_t *init,
                       int32_t *out) {
-        const double t0 = now_s();
+        const int e0 = event();
         patchmatch(dd, pmc, img, sm, w, h, img, tm, w, h, C, init, cfg.seed * 0x9E3779B97F4A7C15ull + (pm_calls++),
                    out, nullptr, pmw, s);
-        NLI_HIP(hipStreamSynchronize(s));
-        t_pm += now_s() - t0;
+        spans.push_back({e0, event(), true});
     };
     auto run_update = [&](float *img, const float *orig, const uint8_t *hole, const int *pts, int npts,
                           const uint8_t *ext, const int32_t *nnf, const float *conf, int w, int h) {
-        const double t0 = now_s();
+        const int e0 = event();
         double total = 0;
         CGStats st{};
         if (cfg.method == UPD_DEPTH) {
@@ -161,7 +177,7 @@ void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask,
             if (cfg.method == UPD_POISSON)
                 for (int c = 0; c < C; c++) rep.cg_iterations_total += st.iterations[c];
         }
-        t_upd += now_s() - t0;
+        spans.push_back({e0, event(), false});
         return total;
     };
 
@@ -236,6 +252,11 @@ void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask,
     NLI_HIP(hipMemcpyAsync(d_out, dw.img[0].p, n0 * C * sizeof(float), hipMemcpyDeviceToDevice, s));
     NLI_HIP(hipStreamSynchronize(s));
     rep.scales = S;
+    for (const Span &sp : spans) {
+        float ms = 0.0f;
+        NLI_HIP(hipEventElapsedTime(&ms, dw.events[sp.a], dw.events[sp.b]));
+        (sp.pm ? t_pm : t_upd) += ms * 1e-3;
+    }
     rep.seconds_total = now_s() - t_start;
     rep.seconds_patchmatch = t_pm;
     rep.seconds_update = t_upd;

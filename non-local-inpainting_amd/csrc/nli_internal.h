@@ -274,6 +274,15 @@ struct DriverWork {
     DBuf<int> dtst, hole_pts, low_pts, counts, total, nonempty;
     DBuf<uint8_t> ext, tgt, src, bscr;
     DBuf<int32_t> nnfA, nnfB, nnfC;
+    // timing events (PatchMatch / update spans), reused across runs
+    std::vector<hipEvent_t> events;
+    DriverWork() = default;
+    DriverWork(const DriverWork &) = delete;
+    DriverWork &operator=(const DriverWork &) = delete;
+    ~DriverWork()
+    {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    }
 };
 
 void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask, int W, int H, int C,
