@@ -264,6 +264,10 @@ int nli_adjoint_aggregate(nli_ctx *ctx, int patch, const double *patches, int n_
  * called by ImageInpainting::process (image_inpainting.cpp:125-153): fills the masked
  * pixels of img (W*H*C, HWC) in place. Host computation (coarsest level only). */
 int nli_poisson_fill(float *img, const uint8_t *mask, int W, int H, int C);
+/* the same fill on the device (img, mask device pointers), bit-identical to
+ * nli_poisson_fill: one workgroup per channel, the in-place Gauss-Seidel sweeps run as
+ * anti-diagonal wavefronts */
+int nli_poisson_fill_dev(nli_ctx *ctx, float *img, const uint8_t *mask, int W, int H, int C);
 
 /* ---- the whole multiscale inpainting ---- */
 /* replaces ImageInpainting::process (image_inpainting.h:54, image_inpainting.cpp:68-190)

@@ -39,7 +39,7 @@ EXPORTS = [
     "nli_dev_alloc", "nli_dev_free", "nli_copy_h2d", "nli_copy_d2h", "nli_copy_d2d",
     "nli_gradient", "nli_distance_transform", "nli_confidence", "nli_sample_size",
     "nli_subsampling_rate", "nli_downsample_with_mask", "nli_downsample", "nli_downsample_mask",
-    "nli_upsample", "nli_extended_domain", "nli_scale_masks", "nli_scale_nnf", "nli_patch_distance",
+    "nli_upsample", "nli_extended_domain", "nli_scale_masks", "nli_scale_nnf", "nli_poisson_fill_dev", "nli_patch_distance",
     "nli_patchmatch", "nli_patchmatch_dev", "nli_update", "nli_update_dev", "nli_poisson_fill",
     "nli_inpaint", "nli_inpaint_dev", "nli_inpaint_params_default", "nli_profile_enable",
     "nli_patch_energy", "nli_energy_gradient", "nli_patch_extract", "nli_adjoint_aggregate",
@@ -472,6 +472,20 @@ class Context:
                                                    patches.shape[0], _ptr(ext, _u8p), _ptr(nnf, _i32p), W, H, C,
                                                    side, out.ctypes.data_as(ct.POINTER(ct.c_double))))
         return out
+
+    def poisson_fill(self, img, mask):
+        """Coarsest-level Poisson fill on the device (bit-identical to the host fill)."""
+        img = _c(img, np.float32)
+        mask = _c(mask, np.uint8)
+        H, W, C = img.shape
+        d_img = self.to_device(img)
+        d_mask = self.to_device(mask)
+        try:
+            self._check(self.lib.nli_poisson_fill_dev(self.h, ct.c_void_p(d_img), ct.c_void_p(d_mask), W, H, C))
+            return self.from_device(d_img, img.shape, np.float32)
+        finally:
+            self.dev_free(d_img)
+            self.dev_free(d_mask)
 
     def copy_d2d(self, dst: int, src: int, nbytes: int):
         self._check(self.lib.nli_copy_d2d(self.h, ct.c_void_p(dst), ct.c_void_p(src), ct.c_uint64(nbytes)))

@@ -781,6 +781,16 @@ int nli_update(nli_ctx *ctx, const nli_upd_params *upd, float *image, const floa
     });
 }
 
+int nli_poisson_fill_dev(nli_ctx *ctx, float *img, const uint8_t *mask, int W, int H, int C)
+{
+    return guard(ctx, [&] {
+        need(img && mask && W > 0 && H > 0 && C >= 1 && C <= 4, "bad poisson fill arguments");
+        float *scr = ctx->f2.ensure(nli::poisson_fill_scratch(W, H, C));
+        nli::poisson_fill_device(img, mask, W, H, C, scr, ctx->stream);
+        sync(ctx);
+    });
+}
+
 int nli_poisson_fill(float *img, const uint8_t *mask, int W, int H, int C)
 {
     if (!img || !mask || W <= 0 || H <= 0 || C <= 0) return NLI_E_ARGUMENT;

@@ -5,7 +5,8 @@
 // The image pyramid, the per-scale masks, the confidence mask, every PatchMatch and
 // every update run on the device; the host keeps the control flow and reads back one
 // scalar per inpainting iteration (the total difference of the stopping rule).  The
-// coarsest level (about 100 x 100 pixels) is filled on the host.
+// coarsest-level Poisson fill runs on the device too; the black and average-colour
+// initialisations run on the host.
 #include "nli_internal.h"
 
 #include <algorithm>
@@ -100,8 +101,12 @@ void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask,
         }
     }
 
-    // ---- coarsest-level initialisation (:104-154), on the host ----
-    if (cfg.init != INIT_NONE) {
+    // ---- coarsest-level initialisation (:104-154) ----
+    if (cfg.init == INIT_POISSON) {
+        const int w = ws[S - 1], h = hs[S - 1];
+        dw.simpois.ensure(poisson_fill_scratch(w, h, C));
+        poisson_fill_device(dw.img[S - 1].p, dw.mask[S - 1].p, w, h, C, dw.simpois.p, s);
+    } else if (cfg.init != INIT_NONE) {   // black / average colour (host, as the reference)
         const int w = ws[S - 1], h = hs[S - 1];
         const size_t n = (size_t)w * h;
         std::vector<float> img(n * C);
@@ -109,25 +114,21 @@ void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask,
         NLI_HIP(hipMemcpyAsync(img.data(), dw.img[S - 1].p, n * C * sizeof(float), hipMemcpyDeviceToHost, s));
         NLI_HIP(hipMemcpyAsync(m.data(), dw.mask[S - 1].p, n, hipMemcpyDeviceToHost, s));
         NLI_HIP(hipStreamSynchronize(s));
-        if (cfg.init == INIT_POISSON) {
-            poisson_fill(img.data(), m.data(), w, h, C);
-        } else {
-            std::vector<float> color(C, 0.0f);
-            if (cfg.init == INIT_AVG)
-                for (int c = 0; c < C; c++) {
-                    float sum = 0.0f;
-                    int count = 0;
-                    for (size_t k = 0; k < n; k++)
-                        if (!m[k]) {
-                            sum += img[k * C + c];
-                            count++;
-                        }
-                    if (count > 0) color[c] = sum / (float)count;
-                }
-            for (size_t k = 0; k < n; k++)
-                if (m[k])
-                    for (int c = 0; c < C; c++) img[k * C + c] = color[c];
-        }
+        std::vector<float> color(C, 0.0f);
+        if (cfg.init == INIT_AVG)
+            for (int c = 0; c < C; c++) {
+                float sum = 0.0f;
+                int count = 0;
+                for (size_t k = 0; k < n; k++)
+                    if (!m[k]) {
+                        sum += img[k * C + c];
+                        count++;
+                    }
+                if (count > 0) color[c] = sum / (float)count;
+            }
+        for (size_t k = 0; k < n; k++)
+            if (m[k])
+                for (int c = 0; c < C; c++) img[k * C + c] = color[c];
         NLI_HIP(hipMemcpyAsync(dw.img[S - 1].p, img.data(), n * C * sizeof(float), hipMemcpyHostToDevice, s));
         NLI_HIP(hipStreamSynchronize(s));
     }

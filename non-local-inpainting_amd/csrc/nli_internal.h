@@ -109,6 +109,10 @@ float subsampling_rate(float ratio, int scales);
 int sample_size(int size, float factor);
 // coarsest-level Poisson fill of the masked pixels (host, in place, HWC)
 void poisson_fill(float *img, const uint8_t *mask, int W, int H, int C);
+// the same on the device (k_simpois.hip): img/mask device pointers, scratch of
+// poisson_fill_scratch(W, H, C) floats; bit-identical to poisson_fill
+size_t poisson_fill_scratch(int W, int H, int C);
+void poisson_fill_device(float *img, const uint8_t *mask, int W, int H, int C, float *scratch, hipStream_t s);
 
 // ---- generic device helpers (k_masks.hip) ----
 // Order-preserving compaction: writes the indices i < n with flag[i] != 0 to out,
@@ -274,6 +278,7 @@ struct DriverWork {
     DBuf<int> dtst, hole_pts, low_pts, counts, total, nonempty;
     DBuf<uint8_t> ext, tgt, src, bscr;
     DBuf<int32_t> nnfA, nnfB, nnfC;
+    DBuf<float> simpois;   // coarsest-level Poisson fill scratch
     // timing events (PatchMatch / update spans), reused across runs
     std::vector<hipEvent_t> events;
     DriverWork() = default;

@@ -42,6 +42,19 @@ def test_distance_transform_and_confidence(ctx, oracle, syn, W, H, seed):
     assert bitexact(ctx.confidence(m, 0.0, 0.3), oracle.confidence(m, 0.0, 0.3))
 
 
+@pytest.mark.parametrize("W,H,seed", [(102, 102, 1), (97, 61, 2), (64, 129, 3), (7, 5, 4)])
+def test_poisson_fill_on_device_is_bitexact(ctx, oracle, syn, W, H, seed):
+    """The coarsest-level fill (simpois.c) on the device: anti-diagonal Gauss-Seidel
+    wavefronts reproduce the sequential in-place sweeps bit for bit."""
+    img = syn.texture(W, H, 3, 3, seed)
+    hole = syn.ellipse_holes(W, H, 3, 0.2, seed, border=1) if min(W, H) > 16 else np.zeros((H, W), np.uint8)
+    if min(W, H) <= 16:
+        hole[1:H - 1, 2:W - 1] = 1
+    hole[0, :W // 3] = 1                                       # touches the image border
+    out = ctx.poisson_fill(img, hole)
+    assert bitexact(out, oracle.simpois(img, hole))
+
+
 def test_golden_numerics(ctx):
     g = golden("numerics")
     img, hole = g["img"], g["hole"]
