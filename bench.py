@@ -87,6 +87,59 @@ def cpu_baseline(wl, gpu_iterations, threads):
                        f"{est:.0f} s = per-scale iteration time x the GPU run's iterations per scale")
 
 
+def bench_rgbd(args, nli, syn):
+    """RGB-D 1920x1080 nldepth images (distinct seeds), one parameter record, inputs
+    resident on the device; --images of them per step (default 8)."""
+    wls = [syn.config_rgbd(seed=2 + k) for k in range(args.images)]
+    ctx = nli.Context(0)
+    q = dict(wls[0].params)
+    q["cg_precision"] = nli.CG_MIXED if args.precision == "mixed" else nli.CG_FP64
+    q["cg_refine_every"] = 8 if args.precision == "mixed" else 0
+    params = nli.Inpainting(**q)
+    H, W, C = wls[0].image.shape
+    d_in = [ctx.to_device(w.image) for w in wls]
+    d_mask = [ctx.to_device(w.mask) for w in wls]
+    d_out = [ctx.dev_alloc(w.image.nbytes) for w in wls]
+    for i in range(args.warmup):
+        t1 = time.perf_counter()
+        ctx.inpaint_batch_dev(d_in, d_mask, d_out, [W] * len(wls), [H] * len(wls), C, params, args.workers)
+        log(f"warmup {i}: {time.perf_counter() - t1:.2f} s")
+    ctx.profile(["pm_sweep"])
+    ctx.lib.nli_synchronize(ctx.h)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        t1 = time.perf_counter()
+        ctx.inpaint_batch_dev(d_in, d_mask, d_out, [W] * len(wls), [H] * len(wls), C, params, args.workers)
+        log(f"step {i}: {time.perf_counter() - t1:.2f} s for {len(wls)} images")
+    ctx.lib.nli_synchronize(ctx.h)
+    elapsed = time.perf_counter() - t0
+    sweep = ctx.profile_read("pm_sweep")
+    ctx.profile(False)
+    outs = [ctx.from_device(d, w.image.shape, np.float32) for d, w in zip(d_out, wls)]
+    rmse = [float(np.sqrt(np.mean((o[w.mask.astype(bool)][:, 3] - w.truth[w.mask.astype(bool)][:, 3]) ** 2)))
+            for o, w in zip(outs, wls)]
+    ms_per_step = elapsed / args.steps * 1e3
+    achieved = sweep["work"] / sweep["ms"] / 1e9
+    line = {
+        "metric": "inpainted MPix/s, RGB-D 1920x1080 nldepth batch",
+        "value": len(wls) * W * H / 1e6 / (ms_per_step / 1e3), "unit": "MPix/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": wls[0].name, "images": len(wls), "width": W, "height": H, "channels": C,
+                   "hole_fraction": float(np.mean([w.mask.mean() for w in wls])), "cg_precision": args.precision,
+                   "workers": args.workers},
+        "roofline": dict(kernel="k_pm_sweep (one PatchMatch sweep)", bound="mfma", achieved=achieved,
+                         peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS, traffic=None,
+                         launches=sweep["launches"], avg_launch_ms=sweep["ms"] / max(sweep["launches"], 1)),
+        "cpu_baseline": None,
+        "depth_rmse_m": float(np.mean(rmse)),
+    }
+    print(json.dumps(line), flush=True)
+    for p in d_in + d_mask + d_out:
+        ctx.dev_free(p)
+    ctx.close()
+
+
 def bench_uneven(args, nli, syn):
     """The uneven batch: 64 images, per-image parameters, inputs resident on the device."""
     t0 = time.perf_counter()
@@ -152,9 +205,10 @@ def main():
                     help="first run 3 device-to-device copies of MIB MiB (the known-size copy that "
                          "calibrates hardware byte counters)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--workload", choices=["rgb1024", "uneven64"], default="rgb1024",
+    ap.add_argument("--workload", choices=["rgb1024", "uneven64", "rgbd1080"], default="rgb1024",
                     help="rgb1024: the headline batch of 1024x1024 nlpoisson images; uneven64: 64 images of "
-                         "sides 128-1024, nlmeans/nlmedians, patch 7/9, 8 scales (per-image parameters)")
+                         "sides 128-1024, nlmeans/nlmedians, patch 7/9, 8 scales (per-image parameters); "
+                         "rgbd1080: 1920x1080 RGB-D images, nldepth (lambda_rgb 1, lambda_d 0), patch 9, 11 scales")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (images are independent: "
@@ -165,6 +219,8 @@ def main():
     syn = importlib.import_module("non-local-inpainting_amd.synthetic")
     if args.workload == "uneven64":
         return bench_uneven(args, nli, syn)
+    if args.workload == "rgbd1080":
+        return bench_rgbd(args, nli, syn)
     wls = [syn.config_1024(seed=1 + k) for k in range(args.images)]
     H, W, C = wls[0].image.shape
     precision = nli.CG_MIXED if args.precision == "mixed" else nli.CG_FP64

@@ -118,8 +118,10 @@ def stripe_holes(width: int, height: int, count: int = 3, stripe_width: int = 40
     y, x = np.mgrid[0:height, 0:width].astype(np.float64)
     m = np.zeros((height, width), dtype=bool)
     for k in range(count):
-        x0 = rng.uniform(0.15, 0.85) * width
-        slope = 0.0 if k == 0 else rng.uniform(-1.2, 1.2)
+        x0 = rng.uniform(0.3, 0.7) * width
+        # the first stripe is vertical, the others diagonal (1.6-2.4 px across per px
+        # down), long enough that three 40 px stripes cover ~12 % of a 1920x1080 image
+        slope = 0.0 if k == 0 else rng.choice([-1.0, 1.0]) * rng.uniform(1.6, 2.4) * height / width * 1920 / 1080
         xc = x0 + slope * (y - height / 2)
         m |= np.abs(x - xc) * math.cos(math.atan(slope)) <= stripe_width / 2
     return m.astype(np.uint8)
