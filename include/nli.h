@@ -207,6 +207,13 @@ int nli_patchmatch_dev(nli_ctx *ctx, const nli_dist_params *dist, const nli_pm_p
                        const uint8_t *tmask, int tW, int tH, int C, const int32_t *init_nnf,
                        uint64_t seed, int32_t *out_nnf, float *out_dist);
 
+/* Tile arrangement of the PatchMatch sweep, process-wide: 0 = autotuned (the default:
+ * the first call for each distance kind, channel count, patch side and size class times
+ * one sweep in each arrangement and keeps the faster), 1 or 2 target tiles per
+ * workgroup = forced.  Fields and distances do not depend on the arrangement.  No
+ * counterpart in the reference. */
+int nli_pm_tiles_per_block(int tiles);
+
 /* ---- image updating ---- */
 /* replaces AImageUpdating::update (a_image_updating.h:34-39) of the four updaters:
  * image (W*H*C) is updated in place on the hole pixels (domain); orig is the image the

@@ -27,6 +27,10 @@
 // memory (L2 / MALL resident), row by row.
 #include "nli_internal.h"
 
+#include <atomic>
+#include <map>
+#include <mutex>
+
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -556,9 +560,9 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
 // the same candidates (neighbours in order left, right, up, down; windows largest first).
 constexpr int kMaxR = 16;                      // random-search windows (max(W, H) < 65536)
 constexpr int kWaves = kBlock / 64;
-constexpr int kTilesPerBlock = 1;
-constexpr int kWaveTargets = kTilesPerBlock * TX;   // targets per wave
+constexpr int kMaxTilesPerBlock = 2;   // tile arrangements the autotuner chooses from: 1 or 2 per block
 static_assert(kWaves == TY, "wave w covers row w of each tile");
+// a block takes `tpb` tiles, so a wave takes tpb * TX targets
 
 __device__ inline int lane_rank(uint64_t mask)
 {
@@ -583,18 +587,19 @@ constexpr uint64_t kNoKey = ((uint64_t)0x7F800000u << 32) | 0xFFFFFFFFu;   // (+
 // floats per pixel, so one pixel's features are 16-byte LDS reads at immediate offsets
 __host__ __device__ constexpr int pool_feat_stride(int F) { return (F + 3) & ~3; }
 __host__ __device__ inline int pool_tasks_per_target(int R) { return R > 4 ? R : 4; }
-__host__ __device__ inline size_t pool_tables_offset(int F, int P)
+__host__ __device__ inline size_t pool_tables_offset(int F, int P, int tpb)
 {
     const int r = P / 2;
-    const size_t tiles = (size_t)kTilesPerBlock * pool_feat_stride(F) * (TX + 2 * r) * (TY + 2 * r);
+    const size_t tiles = (size_t)tpb * pool_feat_stride(F) * (TX + 2 * r) * (TY + 2 * r);
     return ((tiles + (size_t)P * P) * sizeof(float) + 15) & ~(size_t)15;
 }
 // per wave: keys (u64), best (int2), distance (float) per target; candidate (int) and
 // list entry (u16) per task
-__host__ __device__ inline size_t pool_wave_bytes(int R)
+__host__ __device__ inline size_t pool_wave_bytes(int R, int tpb)
 {
-    return (size_t)kWaveTargets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)) +
-           (size_t)kWaveTargets * pool_tasks_per_target(R) * (sizeof(int) + sizeof(uint16_t));
+    const size_t targets = (size_t)tpb * TX;
+    return targets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)) +
+           targets * pool_tasks_per_target(R) * (sizeof(int) + sizeof(uint16_t));
 }
 template <int FP>
 struct TgtLdsI {
@@ -613,10 +618,11 @@ struct WavePool {
 
 // Evaluates the `nvalid` listed tasks of a wave (task j: target j % 32, order j / 32,
 // candidate cand[j]); records each completed candidate in keys[target] by atomic minimum.
-template <int KIND, int C, int P, int FP, bool LAST_FIRST>
+template <int KIND, int C, int P, int FP, bool LAST_FIRST, int TPB>
 __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const float *tiles, int LW, int tile_floats,
                          int wave, const WavePool &wp, int nvalid)
 {
+    constexpr int kWaveTargets = TPB * TX;
     constexpr bool GRAD = Feat<KIND>::grad;
     constexpr int r = P / 2;
     constexpr unsigned PP = (unsigned)(P * P);
@@ -716,9 +722,11 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
     }
 }
 
-template <int KIND, int C, int PS>
+template <int KIND, int C, int PS, int TPB>
 __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
 {
+    constexpr int kTilesPerBlock = TPB;
+    constexpr int kWaveTargets = TPB * TX;
     static_assert(PS > 0 && (KIND == DIST_L2RGBD || KIND == DIST_L2 || KIND == DIST_L1 ||
                              (KIND == DIST_L2COMBINED && (C == 1 || C == 3))),
                   "pool sweep: compile-time patch; gradient distances need the on-the-fly source gradient");
@@ -739,7 +747,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
     const int lw = (int)threadIdx.x & 63;
     WavePool wp;
     {
-        char *pb = reinterpret_cast<char *>(plds) + pool_tables_offset(F, P) + wave * pool_wave_bytes(R);
+        char *pb = reinterpret_cast<char *>(plds) + pool_tables_offset(F, P, TPB) + wave * pool_wave_bytes(R, TPB);
         wp.keys = reinterpret_cast<uint64_t *>(pb);
         wp.best = reinterpret_cast<int2 *>(pb + kWaveTargets * sizeof(uint64_t));
         wp.bd = reinterpret_cast<float *>(pb + kWaveTargets * (sizeof(uint64_t) + sizeof(int2)));
@@ -821,7 +829,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
         nvalid += __popcll(m);
     }
     wave_sync();
-    run_pool<KIND, C, P, FP, false>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);
+    run_pool<KIND, C, P, FP, false, TPB>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);
     wave_sync();
     if (lw < kWaveTargets) {
         const uint64_t k = wp.keys[lw];
@@ -864,7 +872,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
         nvalid += __popcll(m);
     }
     wave_sync();
-    run_pool<KIND, C, P, FP, true>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);   // smallest windows first
+    run_pool<KIND, C, P, FP, true, TPB>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);   // smallest windows first
     wave_sync();
     if (lw < kWaveTargets) {
         int x, y;
@@ -1063,6 +1071,65 @@ void gradient(const float *img, int W, int H, int C, float *grad, hipStream_t s)
     NLI_LAUNCH_CHECK();
 }
 
+namespace {
+
+// ---- autotuning of the pool sweep's tile arrangement ----
+// Choices are kept per process and device for (kind, channels, patch, log2 tile count).
+std::mutex g_tune_mu;
+std::map<std::pair<int, uint64_t>, int> g_tuned;
+std::atomic<int> g_forced_tpb{0};
+
+int current_device()
+{
+    int dev = 0;
+    NLI_HIP(hipGetDevice(&dev));
+    return dev;
+}
+
+int tuned_tiles_per_block(uint64_t key)
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tuned.find({current_device(), key});
+    return it == g_tuned.end() ? 0 : it->second;
+}
+
+// Times one sweep in each arrangement (the sweep is deterministic, so running it twice
+// on the same input writes the same output) and keeps the faster.
+template <class Launch>
+int tune_tiles_per_block(uint64_t key, Launch &&launch, hipStream_t s)
+{
+    hipEvent_t ev[kMaxTilesPerBlock + 1];
+    for (hipEvent_t &e : ev) NLI_HIP(hipEventCreate(&e));
+    NLI_HIP(hipEventRecord(ev[0], s));
+    for (int t = 1; t <= kMaxTilesPerBlock; t++) {
+        launch(t);
+        NLI_HIP(hipEventRecord(ev[t], s));
+    }
+    NLI_HIP(hipEventSynchronize(ev[kMaxTilesPerBlock]));
+    int best = 1;
+    float best_ms = 0.0f;
+    for (int t = 1; t <= kMaxTilesPerBlock; t++) {
+        float ms = 0.0f;
+        NLI_HIP(hipEventElapsedTime(&ms, ev[t - 1], ev[t]));
+        if (t == 1 || ms < best_ms) {
+            best = t;
+            best_ms = ms;
+        }
+    }
+    for (hipEvent_t e : ev) NLI_HIP(hipEventDestroy(e));
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    return g_tuned.emplace(std::make_pair(current_device(), key), best).first->second;
+}
+
+}  // namespace
+
+void set_pm_tiles_per_block(int tpb)
+{
+    if (tpb < 0 || tpb > kMaxTilesPerBlock) throw std::invalid_argument("tiles per block: 0 (auto), 1 or 2");
+    g_forced_tpb = tpb;
+}
+int pm_tiles_per_block() { return g_forced_tpb.load(); }
+
 void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const uint8_t *smask, int sW, int sH,
                 const float *tgt, const uint8_t *tmask, int tW, int tH, int C, const int32_t *init_nnf,
                 uint64_t seed, int32_t *out_nnf, float *out_dist, PMWork &w, hipStream_t s)
@@ -1124,8 +1191,30 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
                                      (d.kind == DIST_L2RGBD && C == 4 && d.patch == 9) ||
                                      ((d.kind == DIST_L2 || d.kind == DIST_L1) && C == 3 &&
                                       (d.patch == 7 || d.patch == 9)));
-    const size_t lds_pool = pool_tables_offset(F, d.patch) + kWaves * pool_wave_bytes(R);
-    const int pool_blocks = ((pr.ntiles + kTilesPerBlock - 1) / kTilesPerBlock + kXCDs - 1) / kXCDs * kXCDs;
+    // launches one candidate-pool sweep with `tpb` tiles per block
+    auto launch_pool = [&](int tpb) {
+        const size_t lds_pool = pool_tables_offset(F, d.patch, tpb) + kWaves * pool_wave_bytes(R, tpb);
+        const int blocks = ((pr.ntiles + tpb - 1) / tpb + kXCDs - 1) / kXCDs * kXCDs;
+#define NLI_POOL(KD, CC, PP)                                                                  \
+    (tpb == 1 ? k_pm_sweep_pool<KD, CC, PP, 1> : k_pm_sweep_pool<KD, CC, PP, 2>)                \
+        <<<blocks, kBlock, lds_pool, s>>>(a)
+        if (d.kind == DIST_L2RGBD) NLI_POOL(DIST_L2RGBD, 4, 9);
+        else if (d.kind == DIST_L2 && d.patch == 7) NLI_POOL(DIST_L2, 3, 7);
+        else if (d.kind == DIST_L2) NLI_POOL(DIST_L2, 3, 9);
+        else if (d.kind == DIST_L1 && d.patch == 7) NLI_POOL(DIST_L1, 3, 7);
+        else if (d.kind == DIST_L1) NLI_POOL(DIST_L1, 3, 9);
+        else if (d.patch == 7) NLI_POOL(DIST_L2COMBINED, 3, 7);
+        else if (d.patch == 9) NLI_POOL(DIST_L2COMBINED, 3, 9);
+        else NLI_POOL(DIST_L2COMBINED, 3, 11);
+#undef NLI_POOL
+        NLI_LAUNCH_CHECK();
+    };
+    // tile arrangement: forced, tuned earlier for this (kind, channels, patch, size), or
+    // tuned now on the first sweep (both arrangements give identical fields)
+    int tpb = pool ? pm_tiles_per_block() : 0;
+    const uint64_t tune_key = ((uint64_t)d.kind << 56) | ((uint64_t)C << 48) | ((uint64_t)d.patch << 40) |
+                              (uint64_t)(32 - __builtin_clz((unsigned)pr.ntiles | 1u));
+    if (pool && tpb == 0) tpb = tuned_tiles_per_block(tune_key);
     for (int k = 0; k < K; k++) {
         a.nnf_in = n_cur;
         a.dist_in = d_cur;
@@ -1133,28 +1222,16 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
         a.dist_out = d_oth;
         a.step = s0 >> (k % (levels + 1));
         a.stream = (uint32_t)k;
+        if (pool && tpb == 0) tpb = tune_tiles_per_block(tune_key, launch_pool, s);
         prof::begin(TK_PM_SWEEP, s);
         if (pool) {
-            if (d.kind == DIST_L2RGBD)
-                k_pm_sweep_pool<DIST_L2RGBD, 4, 9><<<pool_blocks, kBlock, lds_pool, s>>>(a);
-            else if (d.kind == DIST_L2)
-                (d.patch == 7 ? k_pm_sweep_pool<DIST_L2, 3, 7> : k_pm_sweep_pool<DIST_L2, 3, 9>)
-                    <<<pool_blocks, kBlock, lds_pool, s>>>(a);
-            else if (d.kind == DIST_L1)
-                (d.patch == 7 ? k_pm_sweep_pool<DIST_L1, 3, 7> : k_pm_sweep_pool<DIST_L1, 3, 9>)
-                    <<<pool_blocks, kBlock, lds_pool, s>>>(a);
-            else if (d.patch == 7)
-                k_pm_sweep_pool<DIST_L2COMBINED, 3, 7><<<pool_blocks, kBlock, lds_pool, s>>>(a);
-            else if (d.patch == 9)
-                k_pm_sweep_pool<DIST_L2COMBINED, 3, 9><<<pool_blocks, kBlock, lds_pool, s>>>(a);
-            else
-                k_pm_sweep_pool<DIST_L2COMBINED, 3, 11><<<pool_blocks, kBlock, lds_pool, s>>>(a);
+            launch_pool(tpb);
         } else {
 #define NLI_SWEEP(KD, CC, PP) k_pm_sweep<KD, CC, PP><<<sweep_blocks, kBlock, lds, s>>>(a)
             NLI_DISPATCH(d.kind, C, d.patch, NLI_SWEEP);
 #undef NLI_SWEEP
+            NLI_LAUNCH_CHECK();
         }
-        NLI_LAUNCH_CHECK();
         prof::end(TK_PM_SWEEP, s, sweep_flops, pr.targets);
         std::swap(n_cur, n_oth);
         std::swap(d_cur, d_oth);

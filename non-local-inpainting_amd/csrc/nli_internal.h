@@ -174,6 +174,12 @@ struct PMWork {          // per-call scratch, grown on demand
     uint64_t tiles_epoch = ~0ull, epoch = 0;
 };
 
+// Tile arrangement of the candidate-pool sweep: 0 = autotuned (per process, device,
+// distance kind, channels, patch side and size class), 1 or 2 tiles per block = forced.
+// The arrangement does not change results.
+void set_pm_tiles_per_block(int tpb);
+int pm_tiles_per_block();
+
 // The target mask behind a PatchMatch call changed (contents may change behind the
 // same pointer): the cached tile list is rebuilt on the next call.
 inline void pm_masks_changed(PMWork &w) { w.epoch++; }

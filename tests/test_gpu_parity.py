@@ -177,6 +177,33 @@ def test_patchmatch_energy_vs_serial_reference(ctx, oracle, syn, kind, lam):
     assert ratio <= 1.01
 
 
+@pytest.mark.parametrize("kind,C,patch", [("l2combined", 3, 9), ("l2combined", 3, 11), ("l2", 3, 7),
+                                           ("l1", 3, 9), ("l2rgbd", 4, 9)])
+def test_pool_tile_arrangements_give_identical_fields(ctx, syn, nli, kind, C, patch):
+    """The autotuner chooses between 1 and 2 target tiles per workgroup; the fields and
+    distances must not depend on the choice."""
+    W, H = 150, 118
+    img = syn.texture(W, H, 3, 5, 9)
+    if C == 4:
+        img = np.concatenate([img, syn.depth_planes(W, H, 5, 9)[..., None]], axis=2)
+    hole = syn.ellipse_holes(W, H, 4, 0.15, 9)
+    ext, tgt, src = ctx.scale_masks(hole, patch, True)
+    res = []
+    try:
+        for tiles in (1, 2, 0):
+            assert ctx.lib.nli_pm_tiles_per_block(tiles) == 0
+            n, d = ctx.patchmatch(kind, img, src, img, tgt, None, seed=4, patch=patch, lam=0.3, lam_d=0.6,
+                                  with_dist=True)
+            n2, d2 = ctx.patchmatch(kind, img, src, img, tgt, n, seed=5, patch=patch, lam=0.3, lam_d=0.6,
+                                    with_dist=True)
+            res.append((n, d, n2, d2))
+    finally:
+        ctx.lib.nli_pm_tiles_per_block(0)
+    for r in res[1:]:
+        for u, v in zip(res[0], r):
+            assert np.array_equal(u, v)
+
+
 def test_patchmatch_given_field_only_improves(ctx, oracle, syn):
     W = H = 96
     img = syn.texture(W, H, 3, 5, 4)
