@@ -242,11 +242,17 @@ __global__ void k_write_back(float *__restrict__ img, int Cs, int c0, int nc, co
 // Coefficients at pixel q: gather from the centres p around q (row-major, the order in
 // which the reference's scatter reaches q).  The source gradient is formed from the
 // image with gradient.cpp's float forward differences (0 past the last column / row).
-__device__ inline void poisson_coef_at(int qx, int qy, const float *__restrict__ img, int Cs, int c0, int nc,
-                                       const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
-                                       const float *__restrict__ conf, const float *__restrict__ wl, int P,
-                                       float lambda, int W, int H, double *__restrict__ a1, double *__restrict__ a2,
-                                       double *__restrict__ f1, double *__restrict__ F1, double *__restrict__ F2)
+// The loads of CH centres of a patch row are issued together: first their
+// extended-domain flags, NNF entries and confidences, then every valid centre's source
+// samples (colour, right and lower neighbour); then the centres are accumulated in
+// row-major order.  A row costs two dependent memory trips instead of three per centre.
+template <int NC, int CH>
+__device__ inline void poisson_coef_rows(int qx, int qy, const float *__restrict__ img, int Cs, int c0,
+                                         const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
+                                         const float *__restrict__ conf, const float *__restrict__ wl, int P,
+                                         float lambda, int W, int H, double *__restrict__ a1,
+                                         double *__restrict__ a2, double *__restrict__ f1, double *__restrict__ F1,
+                                         double *__restrict__ F2)
 {
     const int h = P / 2;
     const size_t plane = (size_t)W * H;
@@ -254,48 +260,80 @@ __device__ inline void poisson_coef_at(int qx, int qy, const float *__restrict__
     const double lam = (double)lambda;
     const double lam1 = (double)(1 - lambda);
     double A1 = 0.0, A2 = 0.0;
-    double v1[kMaxNC] = {0, 0, 0, 0}, g1[kMaxNC] = {0, 0, 0, 0}, g2[kMaxNC] = {0, 0, 0, 0};
-    for (int py = max(qy - h, 0); py <= min(qy + h, H - 1); py++)
-        for (int px = max(qx - h, 0); px <= min(qx + h, W - 1); px++) {
-            size_t pidx = (size_t)W * py + px;
-            if (!ext[pidx]) continue;
-            int2 nb = nnf[pidx];
-            int n_x = nb.x + qx - px, n_y = nb.y + qy - py;
-            if (n_x < 0 || n_y < 0 || n_x >= W || n_y >= H) continue;
-            float confidence = conf ? conf[pidx] : 1.0f;
-            double weight = (double)(confidence * wl[P * (qy - py + h) + (qx - px + h)]);
-            A1 += weight * lam;
-            A2 += weight * lam1;
-            size_t nidx = (size_t)W * n_y + n_x;
-            const float *iv = img + nidx * Cs + c0;
-            for (int ch = 0; ch < kMaxNC; ch++)
-                if (ch < nc) {
-                    const float gx = n_x + 1 < W ? iv[ch + Cs] - iv[ch] : 0.0f;
-                    const float gy = n_y + 1 < H ? iv[ch + (size_t)W * Cs] - iv[ch] : 0.0f;
-                    v1[ch] += weight * (double)iv[ch] * lam;
-                    g1[ch] += weight * (double)gx * lam1;
-                    g2[ch] += weight * (double)gy * lam1;
+    double v1[NC], g1[NC], g2[NC];
+#pragma unroll
+    for (int ch = 0; ch < NC; ch++) v1[ch] = g1[ch] = g2[ch] = 0.0;
+    const int x_lo = max(qx - h, 0), x_hi = min(qx + h, W - 1);
+    for (int py = max(qy - h, 0); py <= min(qy + h, H - 1); py++) {
+        const size_t row = (size_t)W * py;
+        for (int px0 = x_lo; px0 <= x_hi; px0 += CH) {
+            bool ok[CH];
+            int2 nb[CH];
+            float cf[CH];
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+                const int px = px0 + j;
+                const bool in = px <= x_hi;
+                const size_t pidx = row + (in ? px : x_lo);
+                ok[j] = in && ext[pidx];
+                nb[j] = nnf[pidx];
+                cf[j] = conf ? conf[pidx] : 1.0f;
+            }
+            float v[CH][NC], gx[CH][NC], gy[CH][NC];
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+                const int px = px0 + j;
+                const int n_x = nb[j].x + qx - px, n_y = nb[j].y + qy - py;
+                ok[j] = ok[j] && n_x >= 0 && n_y >= 0 && n_x < W && n_y < H;
+                const bool right = n_x + 1 < W, below = n_y + 1 < H;
+                const float *iv = img + (ok[j] ? (size_t)W * n_y + n_x : 0) * Cs + c0;
+#pragma unroll
+                for (int ch = 0; ch < NC; ch++) {
+                    const float c = iv[ch];
+                    const float r = (ok[j] && right) ? iv[ch + Cs] : 0.0f;
+                    const float d = (ok[j] && below) ? iv[ch + (size_t)W * Cs] : 0.0f;
+                    v[j][ch] = c;
+                    gx[j][ch] = right ? r - c : 0.0f;
+                    gy[j][ch] = below ? d - c : 0.0f;
                 }
+            }
+#pragma unroll
+            for (int j = 0; j < CH; j++) {
+                if (!ok[j]) continue;
+                const int px = px0 + j;
+                const double weight = (double)(cf[j] * wl[P * (qy - py + h) + (qx - px + h)]);
+                A1 += weight * lam;
+                A2 += weight * lam1;
+#pragma unroll
+                for (int ch = 0; ch < NC; ch++) {
+                    v1[ch] += weight * (double)v[j][ch] * lam;
+                    g1[ch] += weight * (double)gx[j][ch] * lam1;
+                    g2[ch] += weight * (double)gy[j][ch] * lam1;
+                }
+            }
         }
+    }
     a1[q] = A1;
     a2[q] = A2;
-    for (int ch = 0; ch < nc; ch++) {
+#pragma unroll
+    for (int ch = 0; ch < NC; ch++) {
         f1[ch * plane + q] = v1[ch];
         F1[ch * plane + q] = g1[ch];
         F2[ch * plane + q] = g2[ch];
     }
 }
 
-// Coefficients where the solve reads them: every hole point, and its left and top
-// neighbours (the stencil's a2 and the divergence's F there).  A neighbour that is itself
-// a hole point is left to its own thread; one that is the neighbour of two hole points
-// is written twice with the same values.
-__global__ void k_poisson_coef(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
-                               const float *__restrict__ img, int Cs, int c0, int nc,
-                               const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
-                               const float *__restrict__ conf, const float *__restrict__ w, int P, float lambda,
-                               int W, int H, double *__restrict__ a1, double *__restrict__ a2,
-                               double *__restrict__ f1, double *__restrict__ F1, double *__restrict__ F2)
+// Coefficients where the solve reads them: every hole point (blockIdx.y 0), and its left
+// (1) and top (2) neighbours (the stencil's a2 and the divergence's F there).  A
+// neighbour that is itself a hole point is left to its own thread; one that is the
+// neighbour of two hole points is written twice with the same values.
+template <int NC, int CH>
+__global__ __launch_bounds__(kBlock) void k_poisson_coef(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
+                               const float *__restrict__ img, int Cs, int c0, const uint8_t *__restrict__ ext,
+                               const int2 *__restrict__ nnf, const float *__restrict__ conf,
+                               const float *__restrict__ w, int P, float lambda, int W, int H,
+                               double *__restrict__ a1, double *__restrict__ a2, double *__restrict__ f1,
+                               double *__restrict__ F1, double *__restrict__ F2)
 {
     __shared__ float wl[kMaxP * kMaxP];
     for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
@@ -303,12 +341,33 @@ __global__ void k_poisson_coef(const int *__restrict__ pts, int n, const int *__
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int p = pts[k];
-    const int x = p % W, y = p / W;
-    poisson_coef_at(x, y, img, Cs, c0, nc, ext, nnf, conf, wl, P, lambda, W, H, a1, a2, f1, F1, F2);
-    if (x > 0 && lidx[p - 1] < 0)
-        poisson_coef_at(x - 1, y, img, Cs, c0, nc, ext, nnf, conf, wl, P, lambda, W, H, a1, a2, f1, F1, F2);
-    if (y > 0 && lidx[p - W] < 0)
-        poisson_coef_at(x, y - 1, img, Cs, c0, nc, ext, nnf, conf, wl, P, lambda, W, H, a1, a2, f1, F1, F2);
+    int x = p % W, y = p / W;
+    if (blockIdx.y == 1) {
+        if (x == 0 || lidx[p - 1] >= 0) return;
+        x -= 1;
+    } else if (blockIdx.y == 2) {
+        if (y == 0 || lidx[p - W] >= 0) return;
+        y -= 1;
+    }
+    poisson_coef_rows<NC, CH>(x, y, img, Cs, c0, ext, nnf, conf, wl, P, lambda, W, H, a1, a2, f1, F1, F2);
+}
+
+template <int NC>
+void launch_poisson_coef(int P, dim3 grid, hipStream_t s, const int *pts, int n, const int *lidx, const float *img,
+                         int Cs, int c0, const uint8_t *ext, const int2 *nnf, const float *conf, const float *w,
+                         float lambda, int W, int H, double *a1, double *a2, double *f1, double *F1, double *F2)
+{
+#define NLI_COEF(CH)                                                                                             \
+    k_poisson_coef<NC, CH><<<grid, kBlock, 0, s>>>(pts, n, lidx, img, Cs, c0, ext, nnf, conf, w, P, lambda, W, H, \
+                                                   a1, a2, f1, F1, F2)
+    switch (P) {
+    case 5: NLI_COEF(5); break;
+    case 7: NLI_COEF(7); break;
+    case 9: NLI_COEF(9); break;
+    case 11: NLI_COEF(11); break;
+    default: NLI_COEF(4); break;
+    }
+#undef NLI_COEF
 }
 
 // backward divergence of (F1, F2) (patch_non_local_poisson.cpp:235-285, with its
@@ -817,9 +876,20 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     k_lidx<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p);
     NLI_LAUNCH_CHECK();
     prof::begin(TK_POISSON_COEF, s);
-    k_poisson_coef<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p, image, Cs, c0, nc, ext, (const int2 *)nnf,
-                                                  conf, w.weights.p, patch, lambda, W, H, w.a1.p, w.a2.p, w.f1.p,
-                                                  w.F1.p, w.F2.p);
+    {
+        const dim3 grid(grid_for(n), 3);
+        auto coef = [&](auto nc_tag) {
+            constexpr int NC = decltype(nc_tag)::value;
+            launch_poisson_coef<NC>(patch, grid, s, hole_pts, n, w.lidx.p, image, Cs, c0, ext, (const int2 *)nnf,
+                                    conf, w.weights.p, lambda, W, H, w.a1.p, w.a2.p, w.f1.p, w.F1.p, w.F2.p);
+        };
+        switch (nc) {
+        case 1: coef(std::integral_constant<int, 1>{}); break;
+        case 2: coef(std::integral_constant<int, 2>{}); break;
+        case 3: coef(std::integral_constant<int, 3>{}); break;
+        default: coef(std::integral_constant<int, 4>{}); break;
+        }
+    }
     NLI_LAUNCH_CHECK();
     prof::end(TK_POISSON_COEF, s, 0.0, (double)n);
     k_divergence<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.F1.p, w.F2.p, nc, W, H, lambda != 1 ? 1 : 0, w.f2.p);
