@@ -53,11 +53,24 @@ def psnr(a, b):
     return float("inf") if mse == 0 else 10 * np.log10(1.0 / mse)
 
 
-def workload_params(nli, precision):
-    return nli.Inpainting(method="nlpoisson", patch=9, sigma=10000.0, lam=0.05, iters=200, tol=0.1, scales=11,
+def workload_params(nli, precision, iters=200, tol=0.1):
+    return nli.Inpainting(method="nlpoisson", patch=9, sigma=10000.0, lam=0.05, iters=iters, tol=tol, scales=11,
                           coarse=0.1, conf_t=5.0, conf_a=0.1, init="poisson", pm_iters=10, pm_shots=30,
                           pm_window=-1, cg_tol=1e-4, cg_limit=100, cg_precision=precision,
                           cg_refine_every=8 if precision == nli.CG_MIXED else 0, seed=1)
+
+
+def host_cpu():
+    """CPU model name and logical CPU count of this host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
 
 
 def cpu_baseline(wl, gpu_iterations, threads):
@@ -81,7 +94,9 @@ def cpu_baseline(wl, gpu_iterations, threads):
         if st.iterations[L] > 0:
             est += st.seconds_iterations[L] / st.iterations[L] * gpu_iterations[L]
     H, W = wl.mask.shape
-    return dict(value=W * H / 1e6 / est, unit="MPix/s", cores=threads, kind="port",
+    model, nproc = host_cpu()
+    return dict(value=W * H / 1e6 / est, unit="MPix/s", cores=threads, kind="port", cpu_model=model,
+                host_logical_cpus=nproc,
                 sample=f"oracle process() of the same 1024x1024 workload with 1 inpainting iteration per scale "
                        f"({sample_s:.1f} s, PatchMatch on {threads} OpenMP threads); estimated full run "
                        f"{est:.0f} s = per-scale iteration time x the GPU run's iterations per scale")
@@ -205,6 +220,9 @@ def main():
                     help="first run 3 device-to-device copies of MIB MiB (the known-size copy that "
                          "calibrates hardware byte counters)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--fixed-iters", type=int, default=10, metavar="K",
+                    help="also time the batch, one image and the plain drop-in with exactly K inpainting "
+                         "iterations per scale (stopping tolerance 0), for like-for-like comparisons; 0 = skip")
     ap.add_argument("--workload", choices=["rgb1024", "uneven64", "rgbd1080"], default="rgb1024",
                     help="rgb1024: the headline batch of 1024x1024 nlpoisson images; uneven64: 64 images of "
                          "sides 128-1024, nlmeans/nlmedians, patch 7/9, 8 scales (per-image parameters); "
@@ -306,8 +324,28 @@ def main():
                        psnr(out64[hole0], wls[0].truth[hole0]),
                        single_seconds=dict(patchmatch=st1.seconds_patchmatch, update=st1.seconds_update,
                                            total=st1.seconds_total),
-                       iterations_per_scale=st1.iterations, cg_iterations_total=st1.cg_iterations_total)
+                       iterations_per_scale=st1.iterations, cg_iterations_total=st1.cg_iterations_total,
+                       scale_sizes=st1.sizes, hole_pixels_per_scale=st1.hole_pixels,
+                       targets_per_scale=st1.target_pixels,
+                       random_search_windows_per_scale=[int(max(w, h)).bit_length() for w, h in st1.sizes])
         log(f"single image: {compare}")
+        if args.fixed_iters > 0:
+            # fixed work: every scale runs exactly K iterations (the stopping tolerance is 0)
+            pf = workload_params(nli, precision, iters=args.fixed_iters, tol=0.0)
+            t1 = time.perf_counter()
+            stf = ctx.inpaint_batch_dev(d_in, d_mask, d_out, Ws, Hs, C, pf, args.workers)
+            fb = len(wls) * W * H / 1e6 / (time.perf_counter() - t1)
+            t1 = time.perf_counter()
+            ctx.inpaint_dev(d_in[0], d_mask[0], d_out[0], W, H, C, pf)
+            fs = W * H / 1e6 / (time.perf_counter() - t1)
+            t1 = time.perf_counter()
+            ctx.inpaint_plain(wls[0].image, wls[0].mask, pf)
+            fp = W * H / 1e6 / (time.perf_counter() - t1)
+            compare["fixed_iterations"] = dict(iterations_per_scale=args.fixed_iters, batched_mpix_s=fb,
+                                               single_image_mpix_s=fs, plain_dropin_mpix_s=fp,
+                                               batch_speedup_vs_plain=fb / fp,
+                                               iterations_run=stf[0].iterations)
+            log(f"fixed iterations: {compare['fixed_iterations']}")
 
     cpu = None
     if not args.no_cpu_baseline:
