@@ -177,6 +177,51 @@ def test_patchmatch_energy_vs_serial_reference(ctx, oracle, syn, kind, lam):
     assert ratio <= 1.01
 
 
+@pytest.mark.parametrize("kind,C,patch,lam,lam_d", [
+    ("l2combined", 3, 7, 0.05, 0.0),      # candidate-pool sweep, other patch sides
+    ("l2combined", 3, 11, 0.5, 0.0),
+    ("l2rgbd", 4, 9, 1.0, 0.0),           # the RGB-D workload's weights
+    ("l2rgbd", 4, 9, 0.3, 0.6),
+    ("l2combined", 1, 5, 0.05, 0.0),      # lockstep sweep: channel counts / patch sides
+    ("l2", 2, 3, 0.0, 0.0),               # without a specialised pool kernel
+    ("l1", 4, 7, 0.0, 0.0),
+    ("l2combined", 3, 13, 0.05, 0.0),
+])
+def test_patchmatch_energy_more_kinds(ctx, oracle, syn, kind, C, patch, lam, lam_d):
+    """Mean NNF energy over 8 seeds against the serial reference's mean over the same
+    seeds, within max(1 %, two standard errors of the ratio); stored distances bit-exact.
+    The seed-to-seed spread differs a lot between cases: on the RGB-D image below the
+    serial reference's own energies vary by about 11 % (relative standard deviation)
+    from seed to seed, so 1 % alone cannot be resolved there with a few seeds."""
+    W, H = 144, 120
+    img = syn.texture(W, H, 3, 6, 2)
+    if C == 4:
+        img = np.concatenate([img, syn.depth_planes(W, H, 5, 2)[..., None]], axis=2)
+    else:
+        img = np.ascontiguousarray(img[..., :C])
+    hole = syn.ellipse_holes(W, H, 5, 0.15, 2)
+    ext, tgt, src, _ = oracle.scale_masks(hole, patch)
+    dp = oracle.dist_params(kind, patch, 10000.0, lam, lam_d)
+    e_gpu, e_cpu = [], []
+    sel = tgt.astype(bool)
+    for seed in range(1, 9):
+        n_gpu, d_gpu = ctx.patchmatch(kind, img, src, img, tgt, None, seed=seed, patch=patch, lam=lam, lam_d=lam_d,
+                                      with_dist=True)
+        n_cpu = oracle.patchmatch(dp, img, src, img, tgt, None, seed=seed)
+        assert np.all(n_gpu[~sel] == -1)
+        assert np.all(src[n_gpu[sel][:, 1], n_gpu[sel][:, 0]] == 1)
+        assert bitexact(d_gpu[sel], oracle.patch_distance_field(dp, img, img, n_gpu, tgt)[sel])
+        e_gpu.append(energy(oracle, dp, img, n_gpu, tgt))
+        e_cpu.append(energy(oracle, dp, img, n_cpu, tgt))
+    g, c = np.array(e_gpu), np.array(e_cpu)
+    ratio = g.mean() / c.mean()
+    se = ratio * np.sqrt(g.var(ddof=1) / len(g) / g.mean() ** 2 + c.var(ddof=1) / len(c) / c.mean() ** 2)
+    print(f"{kind} C={C} P={patch} lambda {lam}/{lam_d}: energy GPU/serial reference = {ratio:.4f} "
+          f"(standard error {se:.4f}; seed spread GPU {g.std(ddof=1) / g.mean():.3f}, "
+          f"reference {c.std(ddof=1) / c.mean():.3f})")
+    assert ratio <= 1 + max(0.01, 2 * se)
+
+
 @pytest.mark.parametrize("kind,C,patch", [("l2combined", 3, 9), ("l2combined", 3, 11), ("l2", 3, 7),
                                            ("l1", 3, 9), ("l2rgbd", 4, 9)])
 def test_pool_tile_arrangements_give_identical_fields(ctx, syn, nli, kind, C, patch):
@@ -233,8 +278,8 @@ def test_patchmatch_rgbd_and_invalid_init(ctx, oracle, syn):
 
 
 # ---------------------------------------------------------------- updates
-def field_case(oracle, syn, W=96, H=80, P=9, seed=1, kind="l2combined"):
-    img = syn.texture(W, H, 3, 3, seed)
+def field_case(oracle, syn, W=96, H=80, P=9, seed=1, kind="l2combined", C=3):
+    img = syn.texture(W, H, C, 3, seed)
     hole = syn.ellipse_holes(W, H, 4, 0.15, seed, border=6)
     ext, tgt, src, _ = oracle.scale_masks(hole, P)
     nnf = oracle.patchmatch(oracle.dist_params(kind, P, 10000.0, 0.05), img, src, img, tgt, None, seed=seed)
@@ -289,6 +334,27 @@ def test_nlpoisson(ctx, oracle, syn, lam, tol, limit):
           f"max |diff| {err:.2e}")
     assert err <= 1e-5
     for c in range(3):
+        assert abs(st.cg_iterations[c] - so.cg_iterations[c]) <= 1
+        assert abs(st.cg_nr0[c] - so.cg_nr0[c]) <= 1e-9 * so.cg_nr0[c]
+    assert abs(ta - tb) <= 1e-4 * max(tb, 1e-12)
+
+
+@pytest.mark.parametrize("C,P", [(1, 5), (2, 13), (4, 7), (3, 11), (3, 3), (1, 15), (4, 9)])
+def test_nlpoisson_channels_and_patches(ctx, oracle, syn, C, P):
+    """The coefficient kernel's channel-count and patch-row specialisations (and its
+    generic form for P = 3, 13, 15) against the oracle, FP64 CG."""
+    img, hole, ext, nnf, conf = field_case(oracle, syn, W=88, H=76, P=P, seed=2, C=C)
+    orig = img.copy()
+    img = img + np.float32(0.02)
+    a, ta, st = ctx.update("nlpoisson", img, orig, hole, ext, nnf, conf, patch=P, lam=0.05, cg_tol=1e-6,
+                           cg_limit=1000, stats=True)
+    b, tb, so = oracle.update("nlpoisson", img, orig, hole, ext, nnf, conf, patch=P, lam=0.05, cg_tol=1e-6,
+                              cg_limit=1000, stats=True)
+    err = np.max(np.abs(a.astype(np.float64) - b))
+    print(f"C={C} P={P}: CG iterations GPU {list(st.cg_iterations[:C])} oracle {list(so.cg_iterations[:C])}, "
+          f"max |diff| {err:.2e}")
+    assert err <= 1e-5
+    for c in range(C):
         assert abs(st.cg_iterations[c] - so.cg_iterations[c]) <= 1
         assert abs(st.cg_nr0[c] - so.cg_nr0[c]) <= 1e-9 * so.cg_nr0[c]
     assert abs(ta - tb) <= 1e-4 * max(tb, 1e-12)
