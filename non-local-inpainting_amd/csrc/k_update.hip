@@ -111,107 +111,173 @@ __device__ inline bool last_block_total(const double *partials, int nb, unsigned
     return true;
 }
 
-// ---------------- NL-means ----------------
-__global__ void k_nlmeans(const float *__restrict__ img, int Cs, int c0, int nc, const int *__restrict__ pts,
-                          int n, const uint8_t *__restrict__ hole, const int2 *__restrict__ nnf,
-                          const float *__restrict__ conf, const float *__restrict__ w, int P, int W, int H,
-                          float *__restrict__ newv, uint8_t *__restrict__ has)
+// ---------------- one wavefront per hole pixel ----------------
+// The window's contributors are gathered by the 64 lanes at once (element e = lane +
+// 64 s, in the reference's order: x outer, y inner), then compacted in that order into
+// LDS with ballots; the order-dependent float sums run over the LDS copy.  A pixel costs
+// one or two dependent memory trips per channel instead of one per window element.
+__device__ inline int wave_lane_rank(uint64_t mask)
 {
-    __shared__ float wl[kMaxP * kMaxP];
-    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
-    __syncthreads();
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    int p = pts[k];
-    int x = p % W, y = p / W;
-    int h = P / 2;
-    int x_a = max(x - h, 0), x_b = min(x + h, W - 1);
-    int y_a = max(y - h, 0), y_b = min(y + h, H - 1);
-    float tv[kMaxNC] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float tw = 0.0f;
-    for (int i = x_a; i <= x_b; i++)
-        for (int j = y_a; j <= y_b; j++) {
-            size_t q = (size_t)W * j + i;
-            int2 c = nnf[q];
-            if (c.x == -1) continue;
-            c.x += x - i;
-            c.y += y - j;
-            if (c.x < 0 || c.y < 0 || c.x >= W || c.y >= H) continue;
-            float wt = wl[P * (j - y_a) + (i - x_a)];
-            if (conf && hole[q]) wt *= conf[q];
-            tw += wt;
-            const float *src = img + ((size_t)W * c.y + c.x) * Cs + c0;
-            for (int ch = 0; ch < kMaxNC; ch++)
-                if (ch < nc) tv[ch] += src[ch] * wt;
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+constexpr int kNLWaves = kBlock / 64;
+
+// contributors of hole pixel p: source index, weight and gather position (-1: none);
+// returns their count m, and writes the weights to lw[pos]
+template <int SLOTS>
+__device__ inline int nl_gather(int p, int W, int H, int P, const uint8_t *__restrict__ hole,
+                                const int2 *__restrict__ nnf, const float *__restrict__ conf, const float *wl,
+                                float *lw, int (&src)[SLOTS], float (&wt)[SLOTS], int (&pos)[SLOTS])
+{
+    const int lane = (int)threadIdx.x & 63;
+    const int x = p % W, y = p / W, h = P / 2;
+    const int x_a = max(x - h, 0), x_b = min(x + h, W - 1);
+    const int y_a = max(y - h, 0), y_b = min(y + h, H - 1);
+    const int nh = y_b - y_a + 1, cnt = (x_b - x_a + 1) * nh;
+    int m = 0;
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int e = lane + 64 * s;
+        bool ok = e < cnt;
+        const int i = x_a + (ok ? e / nh : 0), j = y_a + (ok ? e % nh : 0);
+        const size_t q = (size_t)W * j + i;
+        int2 c = ok ? nnf[q] : make_int2(-1, -1);
+        ok = ok && c.x != -1;
+        c.x += x - i;
+        c.y += y - j;
+        ok = ok && c.x >= 0 && c.y >= 0 && c.x < W && c.y < H;
+        float v = 0.0f;
+        if (ok) {
+            v = wl[P * (j - y_a) + (i - x_a)];
+            if (conf && hole[q]) v *= conf[q];
         }
-    has[k] = tw > 0.0;
-    if (tw > 0.0)
-        for (int ch = 0; ch < nc; ch++) newv[(size_t)k * nc + ch] = tv[ch] / tw;
+        const uint64_t bal = __ballot(ok);
+        pos[s] = ok ? m + wave_lane_rank(bal) : -1;
+        m += __popcll(bal);
+        src[s] = ok ? W * c.y + c.x : 0;
+        wt[s] = v;
+        if (ok) lw[pos[s]] = v;
+    }
+    wave_lds_sync();
+    return m;
 }
 
-// ---------------- NL-medians ----------------
-// Per hole pixel and channel: (value, weight) pairs inserted in value order into this
-// thread's LDS column (equal values keep gather order), then the weights are subtracted
-// from the total in that order until it drops below half (compute_weighted_median).
-__global__ void k_nlmedians(const float *__restrict__ img, int Cs, int c0, int nc, const int *__restrict__ pts,
-                            int n, const uint8_t *__restrict__ hole, const int2 *__restrict__ nnf,
-                            const float *__restrict__ conf, const float *__restrict__ w, int P, int W, int H,
-                            float *__restrict__ newv, uint8_t *__restrict__ has)
+// NL-means (patch_non_local_means.cpp:23-100): weighted average in gather order
+template <int SLOTS>
+__global__ __launch_bounds__(kBlock) void k_nlmeans_w(const float *__restrict__ img, int Cs, int c0, int nc,
+                                                      const int *__restrict__ pts, int n,
+                                                      const uint8_t *__restrict__ hole,
+                                                      const int2 *__restrict__ nnf, const float *__restrict__ conf,
+                                                      const float *__restrict__ w, int P, int W, int H,
+                                                      float *__restrict__ newv, uint8_t *__restrict__ has)
 {
-    extern __shared__ float med_lds[];
-    const int PP = P * P;
-    const int bs = blockDim.x;
-    float *wl = med_lds;                       // PP weights
-    float *sv = med_lds + PP;                  // PP x bs values
-    float *so = sv + (size_t)PP * bs;          // PP x bs weights
-    for (int i = threadIdx.x; i < PP; i += bs) wl[i] = w[i];
+    constexpr int kE = 64 * SLOTS;
+    __shared__ float wl[kMaxP * kMaxP];
+    __shared__ float lw[kNLWaves][kE];
+    __shared__ float lv[kNLWaves][kMaxNC][kE];
+    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
     __syncthreads();
-    int k = blockIdx.x * bs + threadIdx.x;
-    if (k >= n) return;
-    const int t = threadIdx.x;
-    int p = pts[k];
-    int x = p % W, y = p / W;
-    int h = P / 2;
-    int x_a = max(x - h, 0), x_b = min(x + h, W - 1);
-    int y_a = max(y - h, 0), y_b = min(y + h, H - 1);
-    for (int ch = 0; ch < nc; ch++) {
-        float tw = 0.0f;
-        int m = 0;
-        for (int i = x_a; i <= x_b; i++)
-            for (int j = y_a; j <= y_b; j++) {
-                size_t q = (size_t)W * j + i;
-                int2 c = nnf[q];
-                if (c.x == -1) continue;
-                c.x += x - i;
-                c.y += y - j;
-                if (c.x < 0 || c.y < 0 || c.x >= W || c.y >= H) continue;
-                float wt = wl[P * (j - y_a) + (i - x_a)];
-                if (conf && hole[q]) wt *= conf[q];
-                tw += wt;
-                float v = img[((size_t)W * c.y + c.x) * Cs + c0 + ch];
-                int pos = m;
-                while (pos > 0 && sv[(pos - 1) * bs + t] > v) {
-                    sv[pos * bs + t] = sv[(pos - 1) * bs + t];
-                    so[pos * bs + t] = so[(pos - 1) * bs + t];
-                    pos--;
-                }
-                sv[pos * bs + t] = v;
-                so[pos * bs + t] = wt;
-                m++;
-            }
-        if (ch == 0) has[k] = m > 0;
-        if (m == 0) return;
-        float half = tw / 2;
-        float total = tw;
-        float med = 0.0f;
-        for (int e = 0; e < m; e++) {
-            total -= so[e * bs + t];
-            if (total < half) {
-                med = sv[e * bs + t];
-                break;
-            }
+    const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
+    for (int k = blockIdx.x * kNLWaves + wave; k < n; k += gridDim.x * kNLWaves) {   // uniform per wave
+        int src[SLOTS], pos[SLOTS];
+        float wt[SLOTS];
+        const int m = nl_gather<SLOTS>(pts[k], W, H, P, hole, nnf, conf, wl, lw[wave], src, wt, pos);
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++)
+            if (pos[s] >= 0)
+                for (int ch = 0; ch < nc; ch++) lv[wave][ch][pos[s]] = img[(size_t)src[s] * Cs + c0 + ch];
+        wave_lds_sync();
+        // the float sums in gather order, formed by every lane alike
+        float tw = 0.0f, tv[kMaxNC] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int f = 0; f < m; f++) {
+            const float v = lw[wave][f];
+            tw += v;
+#pragma unroll
+            for (int ch = 0; ch < kMaxNC; ch++)
+                if (ch < nc) tv[ch] += lv[wave][ch][f] * v;
         }
-        newv[(size_t)k * nc + ch] = med;
+        if (lane == 0) {
+            has[k] = tw > 0.0;
+            if (tw > 0.0)
+                for (int ch = 0; ch < nc; ch++) newv[(size_t)k * nc + ch] = tv[ch] / tw;
+        }
+        wave_lds_sync();
+    }
+}
+
+// NL-medians (patch_non_local_medians.cpp:23-126): per channel the contributors are
+// ranked by value (equal values in gather order), placed in that order, and the weights
+// subtracted from the total in that order until it drops below half
+template <int SLOTS>
+__global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict__ img, int Cs, int c0, int nc,
+                                                        const int *__restrict__ pts, int n,
+                                                        const uint8_t *__restrict__ hole,
+                                                        const int2 *__restrict__ nnf,
+                                                        const float *__restrict__ conf,
+                                                        const float *__restrict__ w, int P, int W, int H,
+                                                        float *__restrict__ newv, uint8_t *__restrict__ has)
+{
+    constexpr int kE = 64 * SLOTS;
+    __shared__ float wl[kMaxP * kMaxP];
+    __shared__ float lw[kNLWaves][kE];
+    __shared__ float lv[kNLWaves][kE];
+    __shared__ float sv[kNLWaves][kE];
+    __shared__ float sw[kNLWaves][kE];
+    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
+    __syncthreads();
+    const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
+    for (int k = blockIdx.x * kNLWaves + wave; k < n; k += gridDim.x * kNLWaves) {   // uniform per wave
+        int src[SLOTS], pos[SLOTS];
+        float wt[SLOTS];
+        const int m = nl_gather<SLOTS>(pts[k], W, H, P, hole, nnf, conf, wl, lw[wave], src, wt, pos);
+        if (m == 0) {
+            if (lane == 0) has[k] = 0;
+            continue;
+        }
+        float tw = 0.0f;
+        for (int f = 0; f < m; f++) tw += lw[wave][f];
+        const float half = tw / 2;
+        for (int ch = 0; ch < nc; ch++) {
+            float v[SLOTS];
+#pragma unroll
+            for (int s = 0; s < SLOTS; s++) {
+                v[s] = pos[s] >= 0 ? img[(size_t)src[s] * Cs + c0 + ch] : 0.0f;
+                if (pos[s] >= 0) lv[wave][pos[s]] = v[s];
+            }
+            wave_lds_sync();
+            int rank[SLOTS];
+#pragma unroll
+            for (int s = 0; s < SLOTS; s++) rank[s] = 0;
+            for (int f = 0; f < m; f++) {
+                const float u = lv[wave][f];
+#pragma unroll
+                for (int s = 0; s < SLOTS; s++) rank[s] += (u < v[s] || (u == v[s] && f < pos[s])) ? 1 : 0;
+            }
+#pragma unroll
+            for (int s = 0; s < SLOTS; s++)
+                if (pos[s] >= 0) {
+                    sv[wave][rank[s]] = v[s];
+                    sw[wave][rank[s]] = wt[s];
+                }
+            wave_lds_sync();
+            float total = tw, med = 0.0f;
+            for (int e = 0; e < m; e++) {
+                total -= sw[wave][e];
+                if (total < half) {
+                    med = sv[wave][e];
+                    break;
+                }
+            }
+            if (lane == 0) newv[(size_t)k * nc + ch] = med;
+            wave_lds_sync();
+        }
+        if (lane == 0) has[k] = 1;
     }
 }
 
@@ -965,16 +1031,22 @@ double update(int kind, int patch, float sigma, float lambda, const CGParams &cg
         float *newv = (float *)w.x1.p;
         w.pflag.ensure(n);
         prof::begin(TK_NL_GATHER, s);
+        // one wavefront per hole pixel: 64, 128 or 256 window elements per wavefront
+        const int slots = patch * patch <= 64 ? 1 : (patch * patch <= 128 ? 2 : 4);
+        const unsigned blocks = (unsigned)std::min<size_t>(((size_t)n + kNLWaves - 1) / kNLWaves, 16384);
+#define NLI_NL(KERNEL, S)                                                                                     \
+    KERNEL<S><<<blocks, kBlock, 0, s>>>(image, Cs, c0, nc, hole_pts, n, hole, (const int2 *)nnf, conf, w.weights.p, \
+                                        patch, W, H, newv, w.pflag.p)
         if (kind == UPD_MEANS) {
-            k_nlmeans<<<grid_for(n), kBlock, 0, s>>>(image, Cs, c0, nc, hole_pts, n, hole, (const int2 *)nnf, conf,
-                                                     w.weights.p, patch, W, H, newv, w.pflag.p);
+            if (slots == 1) NLI_NL(k_nlmeans_w, 1);
+            else if (slots == 2) NLI_NL(k_nlmeans_w, 2);
+            else NLI_NL(k_nlmeans_w, 4);
         } else {
-            // two LDS columns of P*P floats per thread; 64 threads up to P = 11, 32 above
-            const int bs = (patch <= 11) ? 64 : 32;
-            const size_t lds = ((size_t)patch * patch * (2 * bs + 1)) * sizeof(float);
-            k_nlmedians<<<(n + bs - 1) / bs, bs, lds, s>>>(image, Cs, c0, nc, hole_pts, n, hole, (const int2 *)nnf,
-                                                           conf, w.weights.p, patch, W, H, newv, w.pflag.p);
+            if (slots == 1) NLI_NL(k_nlmedians_w, 1);
+            else if (slots == 2) NLI_NL(k_nlmedians_w, 2);
+            else NLI_NL(k_nlmedians_w, 4);
         }
+#undef NLI_NL
         NLI_LAUNCH_CHECK();
         prof::end(TK_NL_GATHER, s, 0.0, (double)n);
         const int nb = (int)grid_for(n);
