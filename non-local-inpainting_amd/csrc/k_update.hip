@@ -111,39 +111,41 @@ __device__ inline bool last_block_total(const double *partials, int nb, unsigned
     return true;
 }
 
-// ---------------- one wavefront per hole pixel ----------------
-// The window's contributors are gathered by the 64 lanes at once (element e = lane +
-// 64 s, in the reference's order: x outer, y inner), then compacted in that order into
-// LDS with ballots; the order-dependent float sums run over the LDS copy.  A pixel costs
-// one or two dependent memory trips per channel instead of one per window element.
-__device__ inline int wave_lane_rank(uint64_t mask)
-{
-    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
+// ---------------- 16 lanes per hole pixel ----------------
+// A wavefront takes 4 hole pixels, 16 lanes each.  The lanes of a pixel gather its
+// window's contributors together (element e = lane + 16 s, in the reference's order: x
+// outer, y inner) and compact them in that order into the pixel's LDS arrays with
+// ballots.  The order-dependent float sums then run over the LDS copy.  A pixel costs one
+// or two dependent memory trips per channel instead of one per window element, and the
+// sequential parts run for 4 pixels at once.
+constexpr int kNLWaves = kBlock / 64;
+constexpr int kNLG = 16;               // lanes per pixel
+constexpr int kNLPix = 64 / kNLG;      // pixels per wavefront
+
 __device__ inline void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
-constexpr int kNLWaves = kBlock / 64;
 
-// contributors of hole pixel p: source index, weight and gather position (-1: none);
-// returns their count m, and writes the weights to lw[pos]
+// contributors of hole pixel p (live = false: none): source index, weight and gather
+// position (-1: not a contributor); returns their count m and writes the weights to
+// lw[pos].  Every lane of the wavefront calls it.
 template <int SLOTS>
-__device__ inline int nl_gather(int p, int W, int H, int P, const uint8_t *__restrict__ hole,
+__device__ inline int nl_gather(bool live, int p, int W, int H, int P, const uint8_t *__restrict__ hole,
                                 const int2 *__restrict__ nnf, const float *__restrict__ conf, const float *wl,
                                 float *lw, int (&src)[SLOTS], float (&wt)[SLOTS], int (&pos)[SLOTS])
 {
-    const int lane = (int)threadIdx.x & 63;
+    const int lane = (int)threadIdx.x & 63, gl = lane % kNLG, grp = lane / kNLG;
     const int x = p % W, y = p / W, h = P / 2;
     const int x_a = max(x - h, 0), x_b = min(x + h, W - 1);
     const int y_a = max(y - h, 0), y_b = min(y + h, H - 1);
-    const int nh = y_b - y_a + 1, cnt = (x_b - x_a + 1) * nh;
+    const int nh = y_b - y_a + 1, cnt = live ? (x_b - x_a + 1) * nh : 0;
     int m = 0;
 #pragma unroll
     for (int s = 0; s < SLOTS; s++) {
-        const int e = lane + 64 * s;
+        const int e = gl + kNLG * s;
         bool ok = e < cnt;
         const int i = x_a + (ok ? e / nh : 0), j = y_a + (ok ? e % nh : 0);
         const size_t q = (size_t)W * j + i;
@@ -157,9 +159,9 @@ __device__ inline int nl_gather(int p, int W, int H, int P, const uint8_t *__res
             v = wl[P * (j - y_a) + (i - x_a)];
             if (conf && hole[q]) v *= conf[q];
         }
-        const uint64_t bal = __ballot(ok);
-        pos[s] = ok ? m + wave_lane_rank(bal) : -1;
-        m += __popcll(bal);
+        const uint32_t gb = (uint32_t)(__ballot(ok) >> (kNLG * grp)) & ((1u << kNLG) - 1);
+        pos[s] = ok ? m + __popc(gb & ((1u << gl) - 1)) : -1;
+        m += __popc(gb);
         src[s] = ok ? W * c.y + c.x : 0;
         wt[s] = v;
         if (ok) lw[pos[s]] = v;
@@ -177,32 +179,36 @@ __global__ __launch_bounds__(kBlock) void k_nlmeans_w(const float *__restrict__ 
                                                       const float *__restrict__ w, int P, int W, int H,
                                                       float *__restrict__ newv, uint8_t *__restrict__ has)
 {
-    constexpr int kE = 64 * SLOTS;
+    constexpr int kE = kNLG * SLOTS;
     __shared__ float wl[kMaxP * kMaxP];
-    __shared__ float lw[kNLWaves][kE];
-    __shared__ float lv[kNLWaves][kMaxNC][kE];
+    __shared__ float lw[kNLWaves][kNLPix][kE];
+    __shared__ float lv[kNLWaves][kNLPix][kMaxNC][kE];
     for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
     __syncthreads();
     const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
-    for (int k = blockIdx.x * kNLWaves + wave; k < n; k += gridDim.x * kNLWaves) {   // uniform per wave
+    const int gl = lane % kNLG, grp = lane / kNLG;
+    float *mw = lw[wave][grp];
+    for (int base = (blockIdx.x * kNLWaves + wave) * kNLPix; base < n; base += gridDim.x * kNLWaves * kNLPix) {
+        const int k = base + grp;                  // the loop bound is uniform per wavefront
+        const bool live = k < n;
         int src[SLOTS], pos[SLOTS];
         float wt[SLOTS];
-        const int m = nl_gather<SLOTS>(pts[k], W, H, P, hole, nnf, conf, wl, lw[wave], src, wt, pos);
+        const int m = nl_gather<SLOTS>(live, pts[live ? k : base], W, H, P, hole, nnf, conf, wl, mw, src, wt, pos);
 #pragma unroll
         for (int s = 0; s < SLOTS; s++)
             if (pos[s] >= 0)
-                for (int ch = 0; ch < nc; ch++) lv[wave][ch][pos[s]] = img[(size_t)src[s] * Cs + c0 + ch];
+                for (int ch = 0; ch < nc; ch++) lv[wave][grp][ch][pos[s]] = img[(size_t)src[s] * Cs + c0 + ch];
         wave_lds_sync();
-        // the float sums in gather order, formed by every lane alike
+        // the float sums in gather order, formed by every lane of the pixel alike
         float tw = 0.0f, tv[kMaxNC] = {0.0f, 0.0f, 0.0f, 0.0f};
         for (int f = 0; f < m; f++) {
-            const float v = lw[wave][f];
+            const float v = mw[f];
             tw += v;
 #pragma unroll
             for (int ch = 0; ch < kMaxNC; ch++)
-                if (ch < nc) tv[ch] += lv[wave][ch][f] * v;
+                if (ch < nc) tv[ch] += lv[wave][grp][ch][f] * v;
         }
-        if (lane == 0) {
+        if (live && gl == 0) {
             has[k] = tw > 0.0;
             if (tw > 0.0)
                 for (int ch = 0; ch < nc; ch++) newv[(size_t)k * nc + ch] = tv[ch] / tw;
@@ -223,61 +229,61 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
                                                         const float *__restrict__ w, int P, int W, int H,
                                                         float *__restrict__ newv, uint8_t *__restrict__ has)
 {
-    constexpr int kE = 64 * SLOTS;
+    constexpr int kE = kNLG * SLOTS;
     __shared__ float wl[kMaxP * kMaxP];
-    __shared__ float lw[kNLWaves][kE];
-    __shared__ float lv[kNLWaves][kE];
-    __shared__ float sv[kNLWaves][kE];
-    __shared__ float sw[kNLWaves][kE];
+    __shared__ float lw[kNLWaves][kNLPix][kE];
+    __shared__ float lv[kNLWaves][kNLPix][kE];
+    __shared__ float sv[kNLWaves][kNLPix][kE];
+    __shared__ float sw[kNLWaves][kNLPix][kE];
     for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
     __syncthreads();
     const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
-    for (int k = blockIdx.x * kNLWaves + wave; k < n; k += gridDim.x * kNLWaves) {   // uniform per wave
+    const int gl = lane % kNLG, grp = lane / kNLG;
+    float *mw = lw[wave][grp], *mv = lv[wave][grp], *ms = sv[wave][grp], *mo = sw[wave][grp];
+    for (int base = (blockIdx.x * kNLWaves + wave) * kNLPix; base < n; base += gridDim.x * kNLWaves * kNLPix) {
+        const int k = base + grp;
+        const bool live = k < n;
         int src[SLOTS], pos[SLOTS];
         float wt[SLOTS];
-        const int m = nl_gather<SLOTS>(pts[k], W, H, P, hole, nnf, conf, wl, lw[wave], src, wt, pos);
-        if (m == 0) {
-            if (lane == 0) has[k] = 0;
-            continue;
-        }
+        const int m = nl_gather<SLOTS>(live, pts[live ? k : base], W, H, P, hole, nnf, conf, wl, mw, src, wt, pos);
         float tw = 0.0f;
-        for (int f = 0; f < m; f++) tw += lw[wave][f];
+        for (int f = 0; f < m; f++) tw += mw[f];
         const float half = tw / 2;
         for (int ch = 0; ch < nc; ch++) {
             float v[SLOTS];
 #pragma unroll
             for (int s = 0; s < SLOTS; s++) {
                 v[s] = pos[s] >= 0 ? img[(size_t)src[s] * Cs + c0 + ch] : 0.0f;
-                if (pos[s] >= 0) lv[wave][pos[s]] = v[s];
+                if (pos[s] >= 0) mv[pos[s]] = v[s];
             }
             wave_lds_sync();
             int rank[SLOTS];
 #pragma unroll
             for (int s = 0; s < SLOTS; s++) rank[s] = 0;
             for (int f = 0; f < m; f++) {
-                const float u = lv[wave][f];
+                const float u = mv[f];
 #pragma unroll
                 for (int s = 0; s < SLOTS; s++) rank[s] += (u < v[s] || (u == v[s] && f < pos[s])) ? 1 : 0;
             }
 #pragma unroll
             for (int s = 0; s < SLOTS; s++)
                 if (pos[s] >= 0) {
-                    sv[wave][rank[s]] = v[s];
-                    sw[wave][rank[s]] = wt[s];
+                    ms[rank[s]] = v[s];
+                    mo[rank[s]] = wt[s];
                 }
             wave_lds_sync();
             float total = tw, med = 0.0f;
             for (int e = 0; e < m; e++) {
-                total -= sw[wave][e];
+                total -= mo[e];
                 if (total < half) {
-                    med = sv[wave][e];
+                    med = ms[e];
                     break;
                 }
             }
-            if (lane == 0) newv[(size_t)k * nc + ch] = med;
+            if (live && gl == 0 && m > 0) newv[(size_t)k * nc + ch] = med;
             wave_lds_sync();
         }
-        if (lane == 0) has[k] = 1;
+        if (live && gl == 0) has[k] = m > 0;
     }
 }
 
@@ -1031,20 +1037,30 @@ double update(int kind, int patch, float sigma, float lambda, const CGParams &cg
         float *newv = (float *)w.x1.p;
         w.pflag.ensure(n);
         prof::begin(TK_NL_GATHER, s);
-        // one wavefront per hole pixel: 64, 128 or 256 window elements per wavefront
-        const int slots = patch * patch <= 64 ? 1 : (patch * patch <= 128 ? 2 : 4);
-        const unsigned blocks = (unsigned)std::min<size_t>(((size_t)n + kNLWaves - 1) / kNLWaves, 16384);
+        // 16 lanes per hole pixel, SLOTS window elements per lane
+        const int pp = patch * patch;
+        const int slots = pp <= 32 ? 2 : (pp <= 64 ? 4 : (pp <= 96 ? 6 : (pp <= 128 ? 8 : 15)));
+        const size_t per_block = (size_t)kNLWaves * kNLPix;
+        const unsigned blocks = (unsigned)std::min<size_t>(((size_t)n + per_block - 1) / per_block, 16384);
 #define NLI_NL(KERNEL, S)                                                                                     \
     KERNEL<S><<<blocks, kBlock, 0, s>>>(image, Cs, c0, nc, hole_pts, n, hole, (const int2 *)nnf, conf, w.weights.p, \
                                         patch, W, H, newv, w.pflag.p)
         if (kind == UPD_MEANS) {
-            if (slots == 1) NLI_NL(k_nlmeans_w, 1);
-            else if (slots == 2) NLI_NL(k_nlmeans_w, 2);
-            else NLI_NL(k_nlmeans_w, 4);
+            switch (slots) {
+            case 2: NLI_NL(k_nlmeans_w, 2); break;
+            case 4: NLI_NL(k_nlmeans_w, 4); break;
+            case 6: NLI_NL(k_nlmeans_w, 6); break;
+            case 8: NLI_NL(k_nlmeans_w, 8); break;
+            default: NLI_NL(k_nlmeans_w, 15); break;
+            }
         } else {
-            if (slots == 1) NLI_NL(k_nlmedians_w, 1);
-            else if (slots == 2) NLI_NL(k_nlmedians_w, 2);
-            else NLI_NL(k_nlmedians_w, 4);
+            switch (slots) {
+            case 2: NLI_NL(k_nlmedians_w, 2); break;
+            case 4: NLI_NL(k_nlmedians_w, 4); break;
+            case 6: NLI_NL(k_nlmedians_w, 6); break;
+            case 8: NLI_NL(k_nlmedians_w, 8); break;
+            default: NLI_NL(k_nlmedians_w, 15); break;
+            }
         }
 #undef NLI_NL
         NLI_LAUNCH_CHECK();

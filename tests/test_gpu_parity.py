@@ -321,6 +321,31 @@ def test_nlmedians(ctx, oracle, syn):
     assert bitexact(a, b)
 
 
+@pytest.mark.parametrize("C,P", [(1, 3), (4, 5), (2, 7), (3, 11), (1, 13), (3, 15)])
+def test_nl_updates_patch_sides(ctx, oracle, syn, C, P):
+    """NL-means (bit-exact) and NL-medians on a random field (no value ties: bit-exact)
+    for every window-size specialisation of the 16-lanes-per-pixel kernels."""
+    img, hole, ext, nnf, conf = field_case(oracle, syn, W=80, H=72, P=P, seed=3, C=C)
+    orig = img.copy()
+    img = img + np.float32(0.01)
+    a, ta = ctx.update("nlmeans", img, orig, hole, ext, nnf, conf, patch=P)
+    b, tb = oracle.update("nlmeans", img, orig, hole, ext, nnf, conf, patch=P)
+    assert bitexact(a, b)
+    assert abs(ta - tb) <= 1e-12 * max(1.0, tb)
+    rng = np.random.default_rng(P)
+    _, tgt, src, _ = oracle.scale_masks(hole, P)
+    sp = np.argwhere(src)
+    t = np.argwhere(tgt)
+    rnd = np.full_like(nnf, -1)
+    pick = sp[rng.integers(0, len(sp), len(t))]
+    rnd[t[:, 0], t[:, 1], 0] = pick[:, 1]
+    rnd[t[:, 0], t[:, 1], 1] = pick[:, 0]
+    a, ta = ctx.update("nlmedians", img, img, hole, ext, rnd, conf, patch=P)
+    b, tb = oracle.update("nlmedians", img, img, hole, ext, rnd, conf, patch=P)
+    assert bitexact(a, b)
+    assert abs(ta - tb) <= 1e-12 * max(1.0, tb)
+
+
 @pytest.mark.parametrize("lam,tol,limit", [(0.05, 1e-4, 100), (0.5, 1e-6, 1000), (1.0, 1e-6, 50), (0.0, 1e-6, 300)])
 def test_nlpoisson(ctx, oracle, syn, lam, tol, limit):
     img, hole, ext, nnf, conf = field_case(oracle, syn)
