@@ -83,32 +83,22 @@ __device__ inline void block_reduce_store(double (&v)[NC], double *partials, int
         for (int c = 0; c < NC; c++) partials[(size_t)c * nb + blockIdx.x] = v[c];
 }
 
-// After every block stored its partials: the last block to arrive sums them (in block
-// order) and returns true with the totals in tot[] (valid in thread 0).
-template <int NC>
-__device__ inline bool last_block_total(const double *partials, int nb, unsigned *counter, double (&tot)[NC])
+// Total of the nb block partials of a one-value reduction, in a fixed order: thread t
+// sums partials t, t + kBlock, ..., then a tree over the threads.  One block; it runs
+// after the kernel that stored the partials, so no grid-wide fence is needed.
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restrict__ partials, int nb,
+                                                         double *__restrict__ result)
 {
-    __shared__ bool is_last;
-    __threadfence();
-    if (threadIdx.x == 0) is_last = (atomicAdd(counter, 1u) == (unsigned)(nb - 1));
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    __shared__ double sh[NC][kBlock];
-    for (int c = 0; c < NC; c++) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) s += ((volatile const double *)partials)[(size_t)c * nb + b];
-        sh[c][threadIdx.x] = s;
-    }
+    __shared__ double sh[kBlock];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += kBlock) s += partials[b];
+    sh[threadIdx.x] = s;
     __syncthreads();
     for (int off = kBlock / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off)
-            for (int c = 0; c < NC; c++) sh[c][threadIdx.x] += sh[c][threadIdx.x + off];
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
         __syncthreads();
     }
-    for (int c = 0; c < NC; c++) tot[c] = sh[c][0];
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
+    if (threadIdx.x == 0) *result = sh[0];
 }
 
 // ---------------- 16 lanes per hole pixel ----------------
@@ -290,7 +280,7 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
 // write new values into the hole pixels that received some, and sum (prev - new)^2
 __global__ void k_write_back(float *__restrict__ img, int Cs, int c0, int nc, const int *__restrict__ pts, int n,
                              const float *__restrict__ newv, const uint8_t *__restrict__ has,
-                             double *__restrict__ partials, unsigned *counter, double *result)
+                             double *__restrict__ partials)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
@@ -303,10 +293,7 @@ __global__ void k_write_back(float *__restrict__ img, int Cs, int c0, int nc, co
             px[ch] = v;
         }
     }
-    int nb = gridDim.x;
-    block_reduce_store<1>(acc, partials, nb);
-    double tot[1];
-    if (last_block_total<1>(partials, nb, counter, tot) && threadIdx.x == 0) *result = tot[0];
+    block_reduce_store<1>(acc, partials, (int)gridDim.x);
 }
 
 // ---------------- NL-Poisson coefficients ----------------
@@ -793,7 +780,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_b(CGArgs<T> a, int it)
 template <typename T>
 __global__ void k_poisson_write(float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0, int nc,
                                 const int *__restrict__ pts, int n, T *x0, T *x1, const int *__restrict__ done_at,
-                                double *__restrict__ partials, unsigned *counter, double *result)
+                                double *__restrict__ partials)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
@@ -809,10 +796,7 @@ __global__ void k_poisson_write(float *__restrict__ img, const float *__restrict
             img[p * Cs + c0 + ch] = v;
         }
     }
-    int nb = gridDim.x;
-    block_reduce_store<1>(acc, partials, nb);
-    double tot[1];
-    if (last_block_total<1>(partials, nb, counter, tot) && threadIdx.x == 0) *result = tot[0];
+    block_reduce_store<1>(acc, partials, (int)gridDim.x);
 }
 
 inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -935,7 +919,6 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     w.scalars.ensure(kMaxNC * S_COUNT + 8 + (size_t)kMaxNC * (cg.limit + 2));
     w.partials.ensure((size_t)2 * kMaxNC * nb + 8);
     w.flag.ensure(2 + kMaxNC + 2);
-    NLI_HIP(hipMemsetAsync(w.flag.p, 0, 2 * sizeof(int), s));
     fill_i32(w.flag.p + 2, -1, kMaxNC, s);
     double *result = w.scalars.p + kMaxNC * S_COUNT;
     if (n == 0) {
@@ -989,7 +972,6 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
         w.ds1.ensure((size_t)n * nc);
         w.rs.ensure((size_t)n * nc);
     }
-    unsigned *counter = (unsigned *)w.flag.p + 1;
     const int *done_at = w.flag.p + 2;
     if (cg.precision == 0) {
         CGArgs<double> a = cg_args<double>(cg, n, nc, w, st, nbr, w.x0.p, w.x1.p, w.d0.p, w.d1.p, w.r.p);
@@ -1001,7 +983,7 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
         default: throw std::invalid_argument("poisson: 1 to 4 channels");
         }
         k_poisson_write<double><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.x0.p, w.x1.p, done_at,
-                                                      w.partials.p, counter, result);
+                                                      w.partials.p);
     } else {
         CGArgs<float> a = cg_args<float>(cg, n, nc, w, st, nbr, w.xs0.p, w.xs1.p, w.ds0.p, w.ds1.p, w.rs.p);
         switch (nc) {
@@ -1012,8 +994,9 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
         default: throw std::invalid_argument("poisson: 1 to 4 channels");
         }
         k_poisson_write<float><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.xs0.p, w.xs1.p, done_at,
-                                                     w.partials.p, counter, result);
+                                                     w.partials.p);
     }
+    k_sum_partials<<<1, kBlock, 0, s>>>(w.partials.p, nb, result);
     NLI_LAUNCH_CHECK();
     return read_scalar(result, s);
 }
@@ -1068,11 +1051,9 @@ double update(int kind, int patch, float sigma, float lambda, const CGParams &cg
         const int nb = (int)grid_for(n);
         w.scalars.ensure(kMaxNC * S_COUNT + 8);
         w.partials.ensure((size_t)kMaxNC * nb + 8);
-        w.flag.ensure(4);
-        NLI_HIP(hipMemsetAsync(w.flag.p, 0, 4 * sizeof(int), s));
         double *result = w.scalars.p + kMaxNC * S_COUNT;
-        k_write_back<<<nb, kBlock, 0, s>>>(image, Cs, c0, nc, hole_pts, n, newv, w.pflag.p, w.partials.p,
-                                           (unsigned *)w.flag.p + 1, result);
+        k_write_back<<<nb, kBlock, 0, s>>>(image, Cs, c0, nc, hole_pts, n, newv, w.pflag.p, w.partials.p);
+        k_sum_partials<<<1, kBlock, 0, s>>>(w.partials.p, nb, result);
         NLI_LAUNCH_CHECK();
         return read_scalar(result, s);
     }
