@@ -411,11 +411,143 @@ __global__ __launch_bounds__(kBlock) void k_poisson_coef(const int *__restrict__
     poisson_coef_rows<NC, CH>(x, y, img, Cs, c0, ext, nnf, conf, wl, P, lambda, W, H, a1, a2, f1, F1, F2);
 }
 
+// The same coefficients with 16 lanes per point (4 points per wavefront): the lanes
+// gather the point's whole window at once (element e = lane + 16 s, row-major over the
+// window, the scatter's order), compact the contributing centres in that order into LDS
+// (weight c(p) g, colour and forward differences of the source sample), and then lane j
+// forms accumulator j (a1, a2, f1[c], F1[c], F2[c]) by adding its terms in that order:
+// the FP64 sums of poisson_coef_rows, with one dependent memory trip per window instead
+// of two per patch row.  Grid row 0: hole points, 1 and 2: their left and top
+// neighbours that are not hole points.
+template <int NC, int SLOTS>
+__global__ __launch_bounds__(64) void k_poisson_coef16(const int *__restrict__ pts, int n,
+                                                       const int *__restrict__ lidx, const float *__restrict__ img,
+                                                       int Cs, int c0, const uint8_t *__restrict__ ext,
+                                                       const int2 *__restrict__ nnf, const float *__restrict__ conf,
+                                                       const float *__restrict__ w, int P, float lambda, int W, int H,
+                                                       double *__restrict__ a1, double *__restrict__ a2,
+                                                       double *__restrict__ f1, double *__restrict__ F1,
+                                                       double *__restrict__ F2)
+{
+    static_assert(2 + 3 * NC <= kNLG, "one lane per accumulator");
+    constexpr int kE = kNLG * SLOTS;
+    __shared__ float wl[kMaxP * kMaxP];
+    __shared__ float lw[kNLPix][kE];
+    __shared__ float lv[kNLPix][3 * NC][kE];   // colour, x difference, y difference per channel
+    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
+    __syncthreads();
+    const int lane = (int)threadIdx.x, gl = lane % kNLG, grp = lane / kNLG;
+    const int k = blockIdx.x * kNLPix + grp;
+    bool live = k < n;
+    int qx = 0, qy = 0;
+    if (live) {
+        const int p = pts[k];
+        qx = p % W;
+        qy = p / W;
+        if (blockIdx.y == 1) {
+            live = qx > 0 && lidx[p - 1] < 0;
+            qx -= 1;
+        } else if (blockIdx.y == 2) {
+            live = qy > 0 && lidx[p - W] < 0;
+            qy -= 1;
+        }
+    }
+    const int h = P / 2;
+    const int x_lo = max(qx - h, 0), x_hi = min(qx + h, W - 1);
+    const int y_lo = max(qy - h, 0), y_hi = min(qy + h, H - 1);
+    const int nw = x_hi - x_lo + 1, cnt = live ? nw * (y_hi - y_lo + 1) : 0;
+    // every slot's centre data first, then every slot's source samples: two memory trips
+    bool ok[SLOTS];
+    int2 nb[SLOTS];
+    float cf[SLOTS];
+    int wi[SLOTS];
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int e = gl + kNLG * s;
+        const bool in = e < cnt;
+        const int py = y_lo + (in ? e / nw : 0), px = x_lo + (in ? e % nw : 0);
+        const size_t pidx = (size_t)W * py + px;
+        ok[s] = in && ext[pidx];
+        nb[s] = nnf[pidx];
+        cf[s] = conf ? conf[pidx] : 1.0f;
+        nb[s].x += qx - px;
+        nb[s].y += qy - py;
+        wi[s] = P * (qy - py + h) + (qx - px + h);
+    }
+    float v[SLOTS][3 * NC];
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int n_x = nb[s].x, n_y = nb[s].y;
+        ok[s] = ok[s] && n_x >= 0 && n_y >= 0 && n_x < W && n_y < H;
+        const bool right = n_x + 1 < W, below = n_y + 1 < H;
+        const float *iv = img + (ok[s] ? (size_t)W * n_y + n_x : 0) * Cs + c0;
+#pragma unroll
+        for (int ch = 0; ch < NC; ch++) {
+            const float c = iv[ch];
+            const float r = (ok[s] && right) ? iv[ch + Cs] : 0.0f;
+            const float d = (ok[s] && below) ? iv[ch + (size_t)W * Cs] : 0.0f;
+            v[s][ch] = c;
+            v[s][NC + ch] = right ? r - c : 0.0f;
+            v[s][2 * NC + ch] = below ? d - c : 0.0f;
+        }
+    }
+    int m = 0;
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const uint32_t gb = (uint32_t)(__ballot(ok[s]) >> (kNLG * grp)) & ((1u << kNLG) - 1);
+        const int pos = m + __popc(gb & ((1u << gl) - 1));
+        m += __popc(gb);
+        if (ok[s]) {
+            lw[grp][pos] = cf[s] * wl[wi[s]];
+#pragma unroll
+            for (int j = 0; j < 3 * NC; j++) lv[grp][j][pos] = v[s][j];
+        }
+    }
+    wave_lds_sync();
+    if (!live || gl >= 2 + 3 * NC) return;
+    const double lam = (double)lambda;
+    const double lam1 = (double)(1 - lambda);
+    double acc = 0.0;
+    if (gl < 2) {
+        const double f = gl == 0 ? lam : lam1;
+        for (int i = 0; i < m; i++) acc += (double)lw[grp][i] * f;
+    } else {
+        const int j = gl - 2;                    // colour c, then x and y differences
+        const double f = j < NC ? lam : lam1;
+        const float *vals = lv[grp][j];
+        for (int i = 0; i < m; i++) acc += (double)lw[grp][i] * (double)vals[i] * f;
+    }
+    const size_t plane = (size_t)W * H;
+    const size_t q = (size_t)W * qy + qx;
+    if (gl == 0) a1[q] = acc;
+    else if (gl == 1) a2[q] = acc;
+    else if (gl < 2 + NC) f1[(gl - 2) * plane + q] = acc;
+    else if (gl < 2 + 2 * NC) F1[(gl - 2 - NC) * plane + q] = acc;
+    else F2[(gl - 2 - 2 * NC) * plane + q] = acc;
+}
+
 template <int NC>
 void launch_poisson_coef(int P, dim3 grid, hipStream_t s, const int *pts, int n, const int *lidx, const float *img,
                          int Cs, int c0, const uint8_t *ext, const int2 *nnf, const float *conf, const float *w,
                          float lambda, int W, int H, double *a1, double *a2, double *f1, double *F1, double *F2)
 {
+    // Up to 32 k points the 16-lanes-per-point kernel finishes sooner (two memory trips
+    // per point); above that its LDS (one window per 16 lanes) limits the points in
+    // flight, and the thread-per-point kernel is faster.  Both give identical values.
+    if (n < 32768) {
+        const dim3 g16((unsigned)((n + kNLPix - 1) / kNLPix), 3);
+#define NLI_COEF16(S)                                                                                           \
+    k_poisson_coef16<NC, S><<<g16, 64, 0, s>>>(pts, n, lidx, img, Cs, c0, ext, nnf, conf, w, P, lambda, W, H, a1, \
+                                               a2, f1, F1, F2)
+        const int pp = P * P;
+        if (pp <= 32) NLI_COEF16(2);
+        else if (pp <= 64) NLI_COEF16(4);
+        else if (pp <= 96) NLI_COEF16(6);
+        else if (pp <= 128) NLI_COEF16(8);
+        else NLI_COEF16(15);
+#undef NLI_COEF16
+        return;
+    }
 #define NLI_COEF(CH)                                                                                             \
     k_poisson_coef<NC, CH><<<grid, kBlock, 0, s>>>(pts, n, lidx, img, Cs, c0, ext, nnf, conf, w, P, lambda, W, H, \
                                                    a1, a2, f1, F1, F2)

@@ -385,6 +385,31 @@ def test_nlpoisson_channels_and_patches(ctx, oracle, syn, C, P):
     assert abs(ta - tb) <= 1e-4 * max(tb, 1e-12)
 
 
+def test_nlpoisson_large_hole_bitexact_coefficients(ctx, oracle, syn):
+    """Above 32 k hole points the update switches to the thread-per-point coefficient
+    kernel; against the oracle at 37 k points (NNF from the GPU PatchMatch)."""
+    W, H = 520, 460
+    img = syn.texture(W, H, 3, 3, 6)
+    hole = syn.ellipse_holes(W, H, 5, 0.16, 6, border=6)
+    assert hole.sum() >= 32768
+    ext, tgt, src, _ = oracle.scale_masks(hole, 9)
+    nnf = ctx.patchmatch("l2combined", img, src, img, tgt, None, seed=6, lam=0.05)
+    conf = oracle.confidence(hole, 5.0, 0.1)
+    orig = img.copy()
+    img = img + np.float32(0.02)
+    a, ta, st = ctx.update("nlpoisson", img, orig, hole, ext, nnf, conf, lam=0.05, cg_tol=1e-6, cg_limit=1000,
+                           stats=True)
+    b, tb, so = oracle.update("nlpoisson", img, orig, hole, ext, nnf, conf, lam=0.05, cg_tol=1e-6, cg_limit=1000,
+                              stats=True)
+    err = np.max(np.abs(a.astype(np.float64) - b))
+    print(f"{int(hole.sum())} hole points: CG iterations GPU {list(st.cg_iterations[:3])} oracle "
+          f"{list(so.cg_iterations[:3])}, max |diff| {err:.2e}")
+    assert err <= 1e-5
+    for c in range(3):
+        assert abs(st.cg_iterations[c] - so.cg_iterations[c]) <= 1
+        assert abs(st.cg_nr0[c] - so.cg_nr0[c]) <= 1e-9 * so.cg_nr0[c]
+
+
 def test_nlpoisson_golden(ctx):
     g = golden("field64x48")
     a, ta = ctx.update("nlpoisson", g["img"], g["img"], g["hole"], g["ext"], g["pm_l2c_seed42"], g["conf"],
