@@ -40,7 +40,10 @@ constexpr int kMaxNC = 4;
 constexpr int kMaxP = 15;
 
 // scalar slots per channel
-enum { S_NR0 = 0, S_NR, S_RDOTQ, S_ALPHA, S_BETA, S_DONE, S_ITER, S_NPART, S_COUNT };
+// Per-update scalars read back at the end of an update, in one copy: the total squared
+// change, then per channel the CG's stopping iteration, |r_0|^2 and its final |r|^2.
+constexpr int kSumTotal = 0, kSumDone = 1, kSumNr0 = 1 + kMaxNC, kSumNr = 1 + 2 * kMaxNC;
+constexpr int kSummary = 16;   // doubles; the CG history follows
 
 // ---------------- deterministic block reductions ----------------
 // Butterfly sum over the 64 lanes of a wave: every lane ends with the same bits
@@ -275,6 +278,19 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
         }
         if (live && gl == 0) has[k] = m > 0;
     }
+}
+
+// The CG part of an update's summary: per channel the stopping iteration, |r_0|^2 and
+// the final |r|^2 from the device-side history
+__global__ void k_cg_summary(const double *__restrict__ hist, int hist_len, const int *__restrict__ done_at, int nc,
+                             double *__restrict__ out)
+{
+    const int c = threadIdx.x;
+    if (c >= nc) return;
+    const int d = done_at[c];
+    out[kSumDone + c] = (double)d;
+    out[kSumNr0 + c] = hist[(size_t)c * hist_len];
+    out[kSumNr + c] = hist[(size_t)c * hist_len + (d > 0 ? d : 0)];
 }
 
 // write new values into the hole pixels that received some, and sum (prev - new)^2
@@ -681,6 +697,8 @@ struct CGArgs {
     double *prr, *prq;
     double *hist;
     int *done_at;
+    int *poll_host;    // pinned host slots [2][kMaxNC]: stop state after every poll_every iterations
+    int poll_every;
     float tol;
     int limit;
     int refine_every;  // T = float: residual recomputed in FP64 every k iterations (0 = never)
@@ -817,6 +835,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_a(CGArgs<T> a, int it)
             run[c] = !stop;
         }
         any = any || run[c];
+    }
+    // the stop state for the host's convergence poll, written straight to pinned host
+    // memory (no copy in the stream); slot parity follows the poll count
+    if ((it + 1) % a.poll_every == 0 && blockIdx.x == 0 && threadIdx.x == 0 && a.poll_host) {
+        int *slot = a.poll_host + (((it + 1) / a.poll_every - 1) & 1) * kMaxNC;
+#pragma unroll
+        for (int c = 0; c < NC; c++) slot[c] = run[c] ? -1 : (done[c] >= 0 ? done[c] : it);
     }
     if (!any) return;   // uniform over the grid
     double acc[NC];
@@ -955,20 +980,17 @@ void upload_weights(UpdWork &w, int P, float sigma, hipStream_t s)
 
 // Runs the CG on n unknowns x NC channels; returns with x in a.x[done_at & 1].
 template <int NC, typename T>
-void run_cg(const CGParams &cg, int n, UpdWork &w, CGArgs<T> a, hipStream_t s, CGStats *stats)
+void run_cg(const CGParams &cg, int n, UpdWork &w, CGArgs<T> a, hipStream_t s)
 {
     const int nb = (int)grid_for(n);
     k_cg_init<NC, T><<<nb, kBlock, 0, s>>>(a, w.x0.p);
     NLI_LAUNCH_CHECK();
-    // Convergence is polled every `poll` iterations without draining the stream: after
-    // each chunk the stop flags are copied to pinned memory behind an event, and the
-    // host waits for the previous chunk's copy while the current chunk runs (kernels
-    // of a converged channel return at once).
-    const int poll = 8;
-    if (!w.done_host) {
-        NLI_HIP(hipHostMalloc((void **)&w.done_host, sizeof(int) * 2 * kMaxNC, hipHostMallocDefault));
-        for (hipEvent_t &e : w.poll_ev) NLI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    }
+    // Convergence is polled every a.poll_every iterations without draining the stream:
+    // after each chunk k_cg_a has written the stop flags to pinned host memory, an event
+    // marks the chunk, and the host waits for the previous chunk's event while the
+    // current chunk runs (kernels of a converged channel return at once).
+    if (!a.poll_host || a.poll_every < 1) throw std::logic_error("CG poll slots not set");
+    const int poll = a.poll_every;
     // algorithmic bytes of one iteration of the reference recurrence (two matvecs):
     // 2 x 13 words per unknown and channel
     const double it_bytes = 2.0 * 13 * sizeof(T) * NC * (double)n;
@@ -980,8 +1002,7 @@ void run_cg(const CGParams &cg, int n, UpdWork &w, CGArgs<T> a, hipStream_t s, C
         NLI_LAUNCH_CHECK();
         prof::end(TK_CG, s, it_bytes, (double)NC * n);
         if ((it + 1) % poll == 0 && it <= cg.limit) {
-            int *slot = w.done_host + (chunk & 1) * kMaxNC;
-            NLI_HIP(hipMemcpyAsync(slot, a.done_at, sizeof(int) * NC, hipMemcpyDeviceToHost, s));
+            // k_cg_a of this iteration wrote the stop state to slot chunk & 1
             NLI_HIP(hipEventRecord(w.poll_ev[chunk & 1], s));
             if (chunk > 0) {
                 const int prev = (chunk - 1) & 1;
@@ -992,19 +1013,6 @@ void run_cg(const CGParams &cg, int n, UpdWork &w, CGArgs<T> a, hipStream_t s, C
             }
             chunk++;
         }
-    }
-    if (stats) {
-        int done[kMaxNC];
-        NLI_HIP(hipMemcpyAsync(done, a.done_at, sizeof(int) * NC, hipMemcpyDeviceToHost, s));
-        NLI_HIP(hipStreamSynchronize(s));
-        for (int c = 0; c < NC; c++) {
-            stats->iterations[c] = done[c];
-            NLI_HIP(hipMemcpyAsync(&stats->nr0[c], a.hist + (size_t)c * a.hist_len, sizeof(double),
-                                   hipMemcpyDeviceToHost, s));
-            NLI_HIP(hipMemcpyAsync(&stats->nr[c], a.hist + (size_t)c * a.hist_len + std::max(done[c], 0),
-                                   sizeof(double), hipMemcpyDeviceToHost, s));
-        }
-        NLI_HIP(hipStreamSynchronize(s));
     }
 }
 
@@ -1026,8 +1034,10 @@ CGArgs<T> cg_args(const CGParams &cg, int n, int nc, UpdWork &w, const Stencil *
     a.r = r;
     a.prr = w.partials.p;
     a.prq = w.partials.p + (size_t)kMaxNC * a.nb;
-    a.hist = w.scalars.p + kMaxNC * S_COUNT + 8;
+    a.hist = w.scalars.p + kSummary;
     a.done_at = w.flag.p + 2;
+    a.poll_host = w.done_dev;
+    a.poll_every = 8;
     a.tol = cg.tol;
     a.limit = cg.limit;
     a.refine_every = cg.refine_every;
@@ -1048,11 +1058,17 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     w.F2.ensure(np * nc);
     w.f2.ensure(np * nc);
     const int nb = (int)grid_for(std::max(n, 1));
-    w.scalars.ensure(kMaxNC * S_COUNT + 8 + (size_t)kMaxNC * (cg.limit + 2));
+    w.scalars.ensure(kSummary + (size_t)kMaxNC * (cg.limit + 2));
+    if (!w.done_host) {   // CG poll slots: coherent pinned memory the kernels write directly
+        NLI_HIP(hipHostMalloc((void **)&w.done_host, sizeof(int) * 2 * kMaxNC,
+                              hipHostMallocCoherent | hipHostMallocMapped));
+        NLI_HIP(hipHostGetDevicePointer((void **)&w.done_dev, w.done_host, 0));
+        for (hipEvent_t &e : w.poll_ev) NLI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
     w.partials.ensure((size_t)2 * kMaxNC * nb + 8);
     w.flag.ensure(2 + kMaxNC + 2);
     fill_i32(w.flag.p + 2, -1, kMaxNC, s);
-    double *result = w.scalars.p + kMaxNC * S_COUNT;
+    double *result = w.scalars.p + kSumTotal;
     if (n == 0) {
         if (stats)
             for (int c = 0; c < nc; c++) { stats->iterations[c] = 0; stats->nr0[c] = 0; stats->nr[c] = 0; }
@@ -1108,10 +1124,10 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     if (cg.precision == 0) {
         CGArgs<double> a = cg_args<double>(cg, n, nc, w, st, nbr, w.x0.p, w.x1.p, w.d0.p, w.d1.p, w.r.p);
         switch (nc) {
-        case 1: run_cg<1, double>(cg, n, w, a, s, stats); break;
-        case 2: run_cg<2, double>(cg, n, w, a, s, stats); break;
-        case 3: run_cg<3, double>(cg, n, w, a, s, stats); break;
-        case 4: run_cg<4, double>(cg, n, w, a, s, stats); break;
+        case 1: run_cg<1, double>(cg, n, w, a, s); break;
+        case 2: run_cg<2, double>(cg, n, w, a, s); break;
+        case 3: run_cg<3, double>(cg, n, w, a, s); break;
+        case 4: run_cg<4, double>(cg, n, w, a, s); break;
         default: throw std::invalid_argument("poisson: 1 to 4 channels");
         }
         k_poisson_write<double><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.x0.p, w.x1.p, done_at,
@@ -1119,18 +1135,28 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     } else {
         CGArgs<float> a = cg_args<float>(cg, n, nc, w, st, nbr, w.xs0.p, w.xs1.p, w.ds0.p, w.ds1.p, w.rs.p);
         switch (nc) {
-        case 1: run_cg<1, float>(cg, n, w, a, s, stats); break;
-        case 2: run_cg<2, float>(cg, n, w, a, s, stats); break;
-        case 3: run_cg<3, float>(cg, n, w, a, s, stats); break;
-        case 4: run_cg<4, float>(cg, n, w, a, s, stats); break;
+        case 1: run_cg<1, float>(cg, n, w, a, s); break;
+        case 2: run_cg<2, float>(cg, n, w, a, s); break;
+        case 3: run_cg<3, float>(cg, n, w, a, s); break;
+        case 4: run_cg<4, float>(cg, n, w, a, s); break;
         default: throw std::invalid_argument("poisson: 1 to 4 channels");
         }
         k_poisson_write<float><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.xs0.p, w.xs1.p, done_at,
                                                      w.partials.p);
     }
     k_sum_partials<<<1, kBlock, 0, s>>>(w.partials.p, nb, result);
+    k_cg_summary<<<1, 64, 0, s>>>(w.scalars.p + kSummary, cg.limit + 2, done_at, nc, w.scalars.p);
     NLI_LAUNCH_CHECK();
-    return read_scalar(result, s);
+    double sum[kSummary];
+    NLI_HIP(hipMemcpyAsync(sum, w.scalars.p, sizeof(sum), hipMemcpyDeviceToHost, s));
+    NLI_HIP(hipStreamSynchronize(s));
+    if (stats)
+        for (int c = 0; c < nc; c++) {
+            stats->iterations[c] = (int)sum[kSumDone + c];
+            stats->nr0[c] = sum[kSumNr0 + c];
+            stats->nr[c] = sum[kSumNr + c];
+        }
+    return sum[kSumTotal];
 }
 
 }  // namespace
@@ -1181,9 +1207,9 @@ double update(int kind, int patch, float sigma, float lambda, const CGParams &cg
         NLI_LAUNCH_CHECK();
         prof::end(TK_NL_GATHER, s, 0.0, (double)n);
         const int nb = (int)grid_for(n);
-        w.scalars.ensure(kMaxNC * S_COUNT + 8);
+        w.scalars.ensure(kSummary);
         w.partials.ensure((size_t)kMaxNC * nb + 8);
-        double *result = w.scalars.p + kMaxNC * S_COUNT;
+        double *result = w.scalars.p + kSumTotal;
         k_write_back<<<nb, kBlock, 0, s>>>(image, Cs, c0, nc, hole_pts, n, newv, w.pflag.p, w.partials.p);
         k_sum_partials<<<1, kBlock, 0, s>>>(w.partials.p, nb, result);
         NLI_LAUNCH_CHECK();

@@ -202,6 +202,7 @@ struct UpdWork {
     int w_patch = 0;          // cached patch weights
     float w_sigma = 0.0f;
     int *done_host = nullptr; // pinned: pipelined CG convergence polls
+    int *done_dev = nullptr;  // the same memory as the kernels address it
     hipEvent_t poll_ev[2] = {nullptr, nullptr};
     UpdWork() = default;
     UpdWork(const UpdWork &) = delete;
