@@ -199,21 +199,37 @@ __global__ void k_dt_columns(const uint8_t *mask, int W, int H, float *g)
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= W) return;
     const float INF = (float)(unsigned)(W + H);
+    // the loads of 8 rows are issued before the 8 dependent steps that use them
+    constexpr int kR = 8;
     float prev = mask[x] ? INF : 0.0f;
     g[x] = prev;
-    for (int y = 1; y < H; y++) {
-        float v = mask[(size_t)W * y + x] ? (float)((double)prev + 1.0) : 0.0f;
-        g[(size_t)W * y + x] = v;
-        prev = v;
+    for (int y0 = 1; y0 < H; y0 += kR) {
+        uint8_t m[kR];
+#pragma unroll
+        for (int j = 0; j < kR; j++) m[j] = y0 + j < H ? mask[(size_t)W * (y0 + j) + x] : 0;
+#pragma unroll
+        for (int j = 0; j < kR; j++)
+            if (y0 + j < H) {
+                const float v = m[j] ? (float)((double)prev + 1.0) : 0.0f;
+                g[(size_t)W * (y0 + j) + x] = v;
+                prev = v;
+            }
     }
     float below = g[(size_t)W * (H - 1) + x];
-    for (int y = H - 2; y >= 0; y--) {
-        float cur = g[(size_t)W * y + x];
-        if (below < cur) {
-            cur = (float)((double)below + 1.0);
-            g[(size_t)W * y + x] = cur;
-        }
-        below = cur;
+    for (int y0 = H - 2; y0 >= 0; y0 -= kR) {
+        float c[kR];
+#pragma unroll
+        for (int j = 0; j < kR; j++) c[j] = y0 - j >= 0 ? g[(size_t)W * (y0 - j) + x] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < kR; j++)
+            if (y0 - j >= 0) {
+                float cur = c[j];
+                if (below < cur) {
+                    cur = (float)((double)below + 1.0);
+                    g[(size_t)W * (y0 - j) + x] = cur;
+                }
+                below = cur;
+            }
     }
 }
 
@@ -231,32 +247,43 @@ __device__ inline int dt_sep(int i, int u, float gi, float gu)
     return (int)(a / (double)(2 * (u - i)));
 }
 
+// One thread per row.  The envelope stacks s, t (and g at the stacked columns) live in
+// LDS when a row's 3W words fit (LDS_ROWS threads per block), else in the global scratch
+// st: the scan's dependent stack accesses then cost LDS latency, not memory latency.
+template <bool LDS_STACK>
 __global__ void k_dt_rows(const float *g, int W, int H, float *out, int *st)
 {
+    extern __shared__ int dt_lds[];
     int y = blockIdx.x * blockDim.x + threadIdx.x;
     if (y >= H) return;
     const float *gr = g + (size_t)W * y;
-    int *s = st + (size_t)2 * W * y;
+    int *s = LDS_STACK ? dt_lds + (size_t)3 * W * threadIdx.x : st + (size_t)2 * W * y;
     int *t = s + W;
+    float *gs = LDS_STACK ? reinterpret_cast<float *>(t + W) : nullptr;   // g at s[q]
+    auto g_at = [&](int q) { return LDS_STACK ? gs[q] : gr[s[q]]; };
     int q = 0;
     s[0] = 0;
     t[0] = 0;
+    if (LDS_STACK) gs[0] = gr[0];
     for (int u = 1; u < W; u++) {
-        while (q >= 0 && dt_f(t[q], s[q], gr[s[q]]) > dt_f(t[q], u, gr[u])) q--;
+        const float gu = gr[u];
+        while (q >= 0 && dt_f(t[q], s[q], g_at(q)) > dt_f(t[q], u, gu)) q--;
         if (q < 0) {
             q = 0;
             s[0] = u;
+            if (LDS_STACK) gs[0] = gu;
         } else {
-            unsigned w = (unsigned)(1 + dt_sep(s[q], u, gr[s[q]], gr[u]));
+            unsigned w = (unsigned)(1 + dt_sep(s[q], u, g_at(q), gu));
             if (w < (unsigned)W) {
                 q++;
                 s[q] = u;
                 t[q] = (int)w;
+                if (LDS_STACK) gs[q] = gu;
             }
         }
     }
     for (int u = W - 1; u >= 0; u--) {
-        out[(size_t)W * y + u] = dt_f(u, s[q], gr[s[q]]);
+        out[(size_t)W * y + u] = dt_f(u, s[q], g_at(q));
         if (u == t[q]) q--;
     }
 }
@@ -365,7 +392,13 @@ void distance_transform(const uint8_t *mask, int W, int H, float *out, float *g,
 {
     k_dt_columns<<<grid_for(W), kBlock, 0, s>>>(mask, W, H, g);
     NLI_LAUNCH_CHECK();
-    k_dt_rows<<<(H + 63) / 64, 64, 0, s>>>(g, W, H, out, st);
+    const size_t row_bytes = (size_t)3 * W * sizeof(int);
+    const int per_block = (int)std::min<size_t>(64, (64 << 10) / row_bytes);
+    if (per_block >= 1) {
+        k_dt_rows<true><<<(H + per_block - 1) / per_block, per_block, per_block * row_bytes, s>>>(g, W, H, out, st);
+    } else {
+        k_dt_rows<false><<<(H + 63) / 64, 64, 0, s>>>(g, W, H, out, st);
+    }
     NLI_LAUNCH_CHECK();
 }
 
