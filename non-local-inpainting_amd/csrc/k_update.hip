@@ -684,7 +684,9 @@ __global__ void k_cg_rhs(const int *__restrict__ pts, int n, const float *__rest
 //                 written by block 0 of the phase A that detects the stop
 //   prr, prq    = block partial sums of r.r and r.(A d) ([c * nb + block])
 // Every kernel first sums the previous kernel's partials itself, in block order, so
-// all blocks see identical scalars and no grid-wide fence or atomic is needed.
+// all blocks see identical scalars and no grid-wide fence or atomic is needed.  No
+// kernel writes the partial array it reads (phase A reads prr and writes prq, phase B
+// the reverse): a block that starts late must still find the previous kernel's values.
 template <typename T>
 struct CGArgs {
     const Stencil *st;
