@@ -360,15 +360,36 @@ __global__ void k_tile_flags(const uint8_t *tmask, int W, int H, int ntx, int nt
 
 // Initialisation (patch_match.cpp:94-141): the given entry, or uniform draws until one
 // lands in the source (at most `shots`), then its distance.
+// One thread per target of a 16x4 tile; the tile's target features (with the patch
+// halo) are staged in LDS first, as in the sweeps.
 template <int KIND, int C, int PS>
 __global__ __launch_bounds__(kBlock) void k_pm_init(PMArgs a)
 {
-    __shared__ float wl[15 * 15];
-    for (int i = threadIdx.x; i < a.P * a.P; i += blockDim.x) wl[i] = a.w[i];
-    __syncthreads();
+    extern __shared__ float init_lds[];
+    constexpr int F = Feat<KIND>::grad ? 3 * C : C;
+    const int P = PS ? PS : a.P;
+    const int hr = P / 2;
+    const int LW = TX + 2 * hr, LH = TY + 2 * hr;
+    const int plane = LW * LH;
+    float *wl = init_lds + F * plane;
     int tile = a.tiles[blockIdx.x];
-    int x = (tile % a.ntiles_x) * TX + (int)threadIdx.x % TX;
-    int y = (tile / a.ntiles_x) * TY + (int)threadIdx.x / TX;
+    const int x0 = (tile % a.ntiles_x) * TX, y0 = (tile / a.ntiles_x) * TY;
+    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = a.w[i];
+    for (int i = threadIdx.x; i < plane; i += blockDim.x) {
+        const int gx = x0 - hr + i % LW, gy = y0 - hr + i / LW;
+        const bool in = (unsigned)gx < (unsigned)a.tW && (unsigned)gy < (unsigned)a.tH;
+        const size_t q = in ? (size_t)gy * a.tW + gx : 0;
+#pragma unroll
+        for (int c = 0; c < C; c++) init_lds[c * plane + i] = in ? a.tgt[q * C + c] : 0.0f;
+        if constexpr (F > C) {
+#pragma unroll
+            for (int k = 0; k < 2 * C; k++) init_lds[(C + k) * plane + i] = in ? a.tgrad[q * 2 * C + k] : 0.0f;
+        }
+    }
+    __syncthreads();
+    const int lx = (int)threadIdx.x % TX, ly = (int)threadIdx.x / TX;
+    int x = x0 + lx;
+    int y = y0 + ly;
     if (x >= a.tW || y >= a.tH || !is_target(a, x, y)) return;
     size_t p = (size_t)a.tW * y + x;
     Rng rng = make_rng(a.seed, a.stream, (uint32_t)p);
@@ -391,9 +412,8 @@ __global__ __launch_bounds__(kBlock) void k_pm_init(PMArgs a)
             tries++;
         }
     }
-    int r = a.P / 2;
     if (is_source(a, cx, cy)) {
-        TgtGlobal<C> t{a.tgt, a.tgrad, a.tW, x - r, y - r};
+        const TgtLds t{init_lds, plane, LW, lx, ly};
         a.nnf_out[p] = make_int2(cx, cy);
         a.dist_out[p] = patch_distance<KIND, C, PS>(a, wl, cx, cy, t, FLT_MAX);
     }
@@ -1166,8 +1186,12 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     a.nnf_out = n_cur;
     a.dist_out = d_cur;
     a.stream = 0xFFFFu;
+    const int init_r = d.patch / 2;
+    const int init_F = (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) ? 3 * C : C;
+    const size_t init_lds =
+        ((size_t)init_F * (TX + 2 * init_r) * (TY + 2 * init_r) + (size_t)d.patch * d.patch) * sizeof(float);
     prof::begin(TK_PM_INIT, s);
-#define NLI_INIT(KD, CC, PP) k_pm_init<KD, CC, PP><<<pr.ntiles, TX * TY, 0, s>>>(a)
+#define NLI_INIT(KD, CC, PP) k_pm_init<KD, CC, PP><<<pr.ntiles, TX * TY, init_lds, s>>>(a)
     NLI_DISPATCH(d.kind, C, d.patch, NLI_INIT);
 #undef NLI_INIT
     NLI_LAUNCH_CHECK();
