@@ -88,9 +88,12 @@ __device__ inline void block_reduce_store(double (&v)[NC], double *partials, int
 
 // Total of the nb block partials of a one-value reduction, in a fixed order: thread t
 // sums partials t, t + kBlock, ..., then a tree over the threads.  One block; it runs
-// after the kernel that stored the partials, so no grid-wide fence is needed.
+// after the kernel that stored the partials, so no grid-wide fence is needed.  With a
+// CG history (hist != nullptr) it also writes the CG part of the update's summary: per
+// channel the stopping iteration, |r_0|^2 and the final |r|^2.
 __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restrict__ partials, int nb,
-                                                         double *__restrict__ result)
+                                                         double *__restrict__ out, const double *__restrict__ hist,
+                                                         int hist_len, const int *__restrict__ done_at, int nc)
 {
     __shared__ double sh[kBlock];
     double s = 0.0;
@@ -101,7 +104,14 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restric
         if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *result = sh[0];
+    if (threadIdx.x == 0) out[kSumTotal] = sh[0];
+    const int c = threadIdx.x;
+    if (hist && c < nc) {
+        const int d = done_at[c];
+        out[kSumDone + c] = (double)d;
+        out[kSumNr0 + c] = hist[(size_t)c * hist_len];
+        out[kSumNr + c] = hist[(size_t)c * hist_len + (d > 0 ? d : 0)];
+    }
 }
 
 // ---------------- 16 lanes per hole pixel ----------------
@@ -280,18 +290,6 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
     }
 }
 
-// The CG part of an update's summary: per channel the stopping iteration, |r_0|^2 and
-// the final |r|^2 from the device-side history
-__global__ void k_cg_summary(const double *__restrict__ hist, int hist_len, const int *__restrict__ done_at, int nc,
-                             double *__restrict__ out)
-{
-    const int c = threadIdx.x;
-    if (c >= nc) return;
-    const int d = done_at[c];
-    out[kSumDone + c] = (double)d;
-    out[kSumNr0 + c] = hist[(size_t)c * hist_len];
-    out[kSumNr + c] = hist[(size_t)c * hist_len + (d > 0 ? d : 0)];
-}
 
 // write new values into the hole pixels that received some, and sum (prev - new)^2
 __global__ void k_write_back(float *__restrict__ img, int Cs, int c0, int nc, const int *__restrict__ pts, int n,
@@ -577,32 +575,6 @@ void launch_poisson_coef(int P, dim3 grid, hipStream_t s, const int *pts, int n,
 #undef NLI_COEF
 }
 
-// backward divergence of (F1, F2) (patch_non_local_poisson.cpp:235-285, with its
-// boundary cases) at the hole points, the only points the right-hand side reads it at
-__global__ void k_divergence(const int *__restrict__ pts, int n, const double *__restrict__ F1,
-                             const double *__restrict__ F2, int nc, int W, int H, int active,
-                             double *__restrict__ f2)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const size_t plane = (size_t)W * H;
-    const size_t idx = (size_t)pts[k];
-    const int x = (int)(idx % W), y = (int)(idx / W);
-    for (int ch = 0; ch < nc; ch++) {
-        const double *fx = F1 + ch * plane, *fy = F2 + ch * plane;
-        double dv = 0.0;
-        if (active) {
-            if (x == 0) dv = fx[idx];
-            else if (x == W - 1) dv = -fx[idx - 1];
-            else dv = fx[idx] - fx[idx - 1];
-            if (y == 0) dv += fy[idx];
-            else if (y == H - 1) dv += -fy[idx - W];
-            else dv += fy[idx] - fy[idx - W];
-        }
-        f2[ch * plane + idx] = dv;
-    }
-}
-
 // ---------------- CG set-up ----------------
 // stencil per hole point: a_t, a_l, a_b, a_r, sum, a1  (reference coefficient choice:
 // top/left use a2 of that neighbour, bottom/right a2 of the centre; 0 outside the image)
@@ -610,30 +582,12 @@ struct Stencil {
     double at, al, ab, ar, sum, a1;
 };
 
-__global__ void k_lidx(const int *pts, int n, int *lidx)
+// list index of every hole pixel; block 0 also resets the CG's per-channel stop flags
+__global__ void k_lidx(const int *pts, int n, int *lidx, int *done_at)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < kMaxNC) done_at[threadIdx.x] = -1;
     if (k < n) lidx[pts[k]] = k;
-}
-
-__global__ void k_cg_setup(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
-                           const double *__restrict__ a1, const double *__restrict__ a2, int W, int H,
-                           Stencil *__restrict__ st, int4 *__restrict__ nbr)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    int p = pts[k];
-    int x = p % W, y = p / W;
-    Stencil s;
-    int4 nb;
-    if (y != 0) { s.at = a2[p - W]; nb.x = lidx[p - W]; } else { s.at = 0; nb.x = -1; }
-    if (x != 0) { s.al = a2[p - 1]; nb.y = lidx[p - 1]; } else { s.al = 0; nb.y = -1; }
-    if (y != H - 1) { s.ab = a2[p]; nb.z = lidx[p + W]; } else { s.ab = 0; nb.z = -1; }
-    if (x != W - 1) { s.ar = a2[p]; nb.w = lidx[p + 1]; } else { s.ar = 0; nb.w = -1; }
-    s.sum = s.at + s.al + s.ab + s.ar;
-    s.a1 = a1[p];
-    st[k] = s;
-    nbr[k] = nb;
 }
 
 // anisotropic Laplacian of a list-space vector (0 outside the hole) at point k
@@ -646,33 +600,54 @@ __device__ inline double lap(const Stencil &s, const int4 &nb, const double *v, 
     return ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - vc * s.sum;
 }
 
-// b = f1 - f2 - a1 u0 + L(u0) with u0 over the full image; x0 = image - u0
-__global__ void k_cg_rhs(const int *__restrict__ pts, int n, const float *__restrict__ img,
-                         const float *__restrict__ orig, int Cs, int c0, int nc, const double *__restrict__ a1,
-                         const double *__restrict__ a2, const double *__restrict__ f1,
-                         const double *__restrict__ f2, int W, int H, double *__restrict__ b,
-                         double *__restrict__ x0)
+
+// The solve's set-up at hole point k, in one pass: the backward divergence f2 of (F1, F2)
+// at the point (patch_non_local_poisson.cpp:235-285, with its boundary cases), the
+// stencil and neighbour indices (reference coefficient choice: top/left use a2 of that
+// neighbour, bottom/right a2 of the centre; 0 outside the image), the right-hand side
+// b = f1 - f2 - a1 u0 + L(u0) with u0 over the full image, and x0 = image - u0.
+__global__ void k_cg_prepare(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
+                             const float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0, int nc,
+                             const double *__restrict__ a1, const double *__restrict__ a2,
+                             const double *__restrict__ f1, const double *__restrict__ F1,
+                             const double *__restrict__ F2, int div_active, int W, int H,
+                             Stencil *__restrict__ st, int4 *__restrict__ nbr, double *__restrict__ b,
+                             double *__restrict__ x0)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    int p = pts[k];
-    int x = p % W, y = p / W;
+    const int p = pts[k];
+    const int x = p % W, y = p / W;
     const size_t plane = (size_t)W * H;
-    double at, al, ab, ar;
-    at = (y != 0) ? a2[p - W] : 0.0;
-    al = (x != 0) ? a2[p - 1] : 0.0;
-    ab = (y != H - 1) ? a2[p] : 0.0;
-    ar = (x != W - 1) ? a2[p] : 0.0;
+    Stencil s;
+    int4 nb;
+    if (y != 0) { s.at = a2[p - W]; nb.x = lidx[p - W]; } else { s.at = 0; nb.x = -1; }
+    if (x != 0) { s.al = a2[p - 1]; nb.y = lidx[p - 1]; } else { s.al = 0; nb.y = -1; }
+    if (y != H - 1) { s.ab = a2[p]; nb.z = lidx[p + W]; } else { s.ab = 0; nb.z = -1; }
+    if (x != W - 1) { s.ar = a2[p]; nb.w = lidx[p + 1]; } else { s.ar = 0; nb.w = -1; }
+    s.sum = s.at + s.al + s.ab + s.ar;
+    s.a1 = a1[p];
+    st[k] = s;
+    nbr[k] = nb;
     for (int ch = 0; ch < nc; ch++) {
+        const double *fx = F1 + ch * plane, *fy = F2 + ch * plane;
+        double dv = 0.0;
+        if (div_active) {
+            if (x == 0) dv = fx[p];
+            else if (x == W - 1) dv = -fx[p - 1];
+            else dv = fx[p] - fx[p - 1];
+            if (y == 0) dv += fy[p];
+            else if (y == H - 1) dv += -fy[p - W];
+            else dv += fy[p] - fy[p - W];
+        }
         auto U = [&](int q) { return (double)orig[(size_t)q * Cs + c0 + ch]; };
-        double ut = (y != 0) ? U(p - W) : 0.0;
-        double ul = (x != 0) ? U(p - 1) : 0.0;
-        double ub = (y != H - 1) ? U(p + W) : 0.0;
-        double ur = (x != W - 1) ? U(p + 1) : 0.0;
-        double u0 = U(p);
-        double L = ut * at + ul * al + ub * ab + ur * ar - u0 * (at + al + ab + ar);
-        double rhs = f1[ch * plane + p] + (-f2[ch * plane + p] - a1[p] * u0 + L);
-        b[(size_t)ch * n + k] = rhs;
+        const double ut = (y != 0) ? U(p - W) : 0.0;
+        const double ul = (x != 0) ? U(p - 1) : 0.0;
+        const double ub = (y != H - 1) ? U(p + W) : 0.0;
+        const double ur = (x != W - 1) ? U(p + 1) : 0.0;
+        const double u0 = U(p);
+        const double L = ut * s.at + ul * s.al + ub * s.ab + ur * s.ar - u0 * (s.at + s.al + s.ab + s.ar);
+        b[(size_t)ch * n + k] = f1[ch * plane + p] + (-dv - s.a1 * u0 + L);
         x0[(size_t)ch * n + k] = (double)img[(size_t)p * Cs + c0 + ch] - u0;
     }
 }
@@ -1058,7 +1033,6 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     w.f1.ensure(np * nc);
     w.F1.ensure(np * nc);
     w.F2.ensure(np * nc);
-    w.f2.ensure(np * nc);
     const int nb = (int)grid_for(std::max(n, 1));
     w.scalars.ensure(kSummary + (size_t)kMaxNC * (cg.limit + 2));
     if (!w.done_host) {   // CG poll slots: coherent pinned memory the kernels write directly
@@ -1069,7 +1043,6 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     }
     w.partials.ensure((size_t)2 * kMaxNC * nb + 8);
     w.flag.ensure(2 + kMaxNC + 2);
-    fill_i32(w.flag.p + 2, -1, kMaxNC, s);
     double *result = w.scalars.p + kSumTotal;
     if (n == 0) {
         if (stats)
@@ -1078,7 +1051,7 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     }
     w.lidx.ensure(np);
     fill_i32(w.lidx.p, -1, np, s);
-    k_lidx<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p);
+    k_lidx<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p, w.flag.p + 2);
     NLI_LAUNCH_CHECK();
     prof::begin(TK_POISSON_COEF, s);
     {
@@ -1097,18 +1070,15 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     }
     NLI_LAUNCH_CHECK();
     prof::end(TK_POISSON_COEF, s, 0.0, (double)n);
-    k_divergence<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.F1.p, w.F2.p, nc, W, H, lambda != 1 ? 1 : 0, w.f2.p);
-    NLI_LAUNCH_CHECK();
     w.stencil.ensure((size_t)n * (sizeof(Stencil) / sizeof(double)));
     w.nbr.ensure((size_t)n * 4);
     Stencil *st = (Stencil *)w.stencil.p;
     int4 *nbr = (int4 *)w.nbr.p;
-    k_cg_setup<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p, w.a1.p, w.a2.p, W, H, st, nbr);
-    NLI_LAUNCH_CHECK();
     w.b.ensure((size_t)n * nc);
     w.x0.ensure((size_t)n * nc);
-    k_cg_rhs<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, image, orig, Cs, c0, nc, w.a1.p, w.a2.p, w.f1.p, w.f2.p, W,
-                                            H, w.b.p, w.x0.p);
+    k_cg_prepare<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p, image, orig, Cs, c0, nc, w.a1.p, w.a2.p,
+                                                w.f1.p, w.F1.p, w.F2.p, lambda != 1 ? 1 : 0, W, H, st, nbr, w.b.p,
+                                                w.x0.p);
     NLI_LAUNCH_CHECK();
     if (cg.precision == 0) {
         w.x1.ensure((size_t)n * nc);
@@ -1146,8 +1116,8 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
         k_poisson_write<float><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.xs0.p, w.xs1.p, done_at,
                                                      w.partials.p);
     }
-    k_sum_partials<<<1, kBlock, 0, s>>>(w.partials.p, nb, result);
-    k_cg_summary<<<1, 64, 0, s>>>(w.scalars.p + kSummary, cg.limit + 2, done_at, nc, w.scalars.p);
+    k_sum_partials<<<1, kBlock, 0, s>>>(w.partials.p, nb, w.scalars.p, w.scalars.p + kSummary, cg.limit + 2,
+                                        done_at, nc);
     NLI_LAUNCH_CHECK();
     double sum[kSummary];
     NLI_HIP(hipMemcpyAsync(sum, w.scalars.p, sizeof(sum), hipMemcpyDeviceToHost, s));
@@ -1213,7 +1183,7 @@ double update(int kind, int patch, float sigma, float lambda, const CGParams &cg
         w.partials.ensure((size_t)kMaxNC * nb + 8);
         double *result = w.scalars.p + kSumTotal;
         k_write_back<<<nb, kBlock, 0, s>>>(image, Cs, c0, nc, hole_pts, n, newv, w.pflag.p, w.partials.p);
-        k_sum_partials<<<1, kBlock, 0, s>>>(w.partials.p, nb, result);
+        k_sum_partials<<<1, kBlock, 0, s>>>(w.partials.p, nb, w.scalars.p, nullptr, 0, nullptr, 0);
         NLI_LAUNCH_CHECK();
         return read_scalar(result, s);
     }
