@@ -216,6 +216,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the single-image and FP64 runs")
     ap.add_argument("--no-breakdown", action="store_true", help="skip the all-kernel timing batch")
+    ap.add_argument("--no-colour", action="store_true", help="skip the sRGB <-> Lab conversion timing")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run 3 device-to-device copies of MIB MiB (the known-size copy that "
                          "calibrates hardware byte counters)")
@@ -347,6 +348,38 @@ def main():
                                                iterations_run=stf[0].iterations)
             log(f"fixed iterations: {compare['fixed_iterations']}")
 
+    colour = None
+    if not args.no_colour:
+        # the colour path of the reference's command-line program (0-255 RGB <-> L*a*b*
+        # around process(), main.cpp:117, 307): both conversions over the batch's images
+        # (scaled to 0-255), device-resident, event-timed; not part of the timed steps
+        d_rgb = [ctx.to_device((w.image * 255.0).astype(np.float32)) for w in wls]
+        d_lab = ctx.dev_alloc(W * H * C * 4)
+        d_back = ctx.dev_alloc(W * H * C * 4)
+        for d in d_rgb[:1]:
+            ctx.rgb_to_lab_dev(d, d_lab, W, H)
+            ctx.lab_to_rgb_dev(d_lab, d_back, W, H)
+        res = {}
+        for name, fn, src, dst in (("rgb_to_lab", ctx.rgb_to_lab_dev, None, d_lab),
+                                   ("lab_to_rgb", ctx.lab_to_rgb_dev, d_lab, d_back)):
+            ctx.profile(["color"])
+            for rep in range(4):
+                for d in d_rgb:
+                    fn(src if src is not None else d, dst, W, H)
+            k = ctx.profile_read("color")
+            ctx.profile(False)
+            us = k["ms"] / max(k["launches"], 1) * 1e3
+            gbs = k["work"] / k["ms"] / 1e6
+            res[name] = dict(launches=k["launches"], avg_launch_us=us, bytes_per_launch=k["work"] / k["launches"],
+                             achieved_gb_s=gbs, frac_of_hbm_peak=gbs / HBM_PEAK_GBS)
+        for p in d_rgb + [d_lab, d_back]:
+            ctx.dev_free(p)
+        colour = dict(image=f"{W}x{H}x3 float32, 0-255", kernels=res,
+                      note="24 algorithmic bytes per pixel (3 floats in, 3 out); each of the reference's powf "
+                           "calls (3 per pixel per direction) is evaluated to the correctly rounded float in FP64, "
+                           "which bounds the kernels rather than HBM")
+        log(f"colour path: {colour}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log(f"CPU baseline sample on {args.cpu_threads} threads ...")
@@ -366,6 +399,7 @@ def main():
         "psnr_hole_db": float(np.mean(psnrs)),
         "kernels": kern,
         "compare": compare,
+        "colour_path": colour,
     }
     print(json.dumps(line), flush=True)
     for p in d_in + d_mask + d_out:

@@ -298,6 +298,7 @@ int nli_inpaint_dev(nli_ctx *ctx, const nli_inpaint_params *params, const float 
 #define NLI_KERNEL_CG_ITERATION 2
 #define NLI_KERNEL_POISSON_COEF 3
 #define NLI_KERNEL_NL_GATHER 4
+#define NLI_KERNEL_COLOR 6       /* sRGB <-> Lab conversions; work = bytes */
 #define NLI_PROFILE_ALL (-1)
 int nli_profile_enable(nli_ctx *ctx, int kernels);
 int nli_profile_read(nli_ctx *ctx, int kernel, int *launches, double *total_ms, double *work,
@@ -330,6 +331,44 @@ int nli_inpaint_batch_each_dev(nli_ctx *ctx, const nli_inpaint_params *params, i
 
 /* fills params with the reference CLI defaults (main.cpp:68-82, 220-233, 254, 266) */
 void nli_inpaint_params_default(nli_inpaint_params *p);
+
+/* ---- the colour path around process() ----
+ * The reference's command-line program converts its 0-255 RGB input to CIE L*a*b*
+ * before ImageInpainting::process and the result back afterwards (main.cpp:117, 307).
+ * Conversions run one pixel per thread with the reference's float/double operation
+ * order; each of its powf calls gives the correctly rounded power (equal to glibc's
+ * powf except for a few inputs in ten thousand, where the two differ by one unit in the
+ * last place). */
+/* replaces IOUtility::rgb_to_lab (io_utility.h:51, io_utility.cpp:268-289): for C = 3,
+ * lab receives W*H*3 L*a*b* values; any other C is returned unchanged (lab receives a
+ * copy of the W*H*C input), as the reference returns its argument. */
+int nli_rgb_to_lab(nli_ctx *ctx, const float *rgb, int W, int H, int C, float *lab);
+/* replaces IOUtility::lab_to_rgb (io_utility.h:52, io_utility.cpp:292-318): for C = 3 or
+ * 4, rgb receives W*H*3 values converted from the first three channels; any other C is
+ * returned unchanged (a copy of the W*H*C input).  For C = 4 the reference reads the
+ * colour planes through a pointer into a local image that is released when its branch
+ * ends (io_utility.cpp:303-308), so its result there is undefined; here it is the
+ * conversion of channels 0-2. */
+int nli_lab_to_rgb(nli_ctx *ctx, const float *lab, int W, int H, int C, float *rgb);
+/* device-pointer forms of the two (return when the conversion has completed) */
+int nli_rgb_to_lab_dev(nli_ctx *ctx, const float *rgb, int W, int H, int C, float *lab);
+int nli_lab_to_rgb_dev(nli_ctx *ctx, const float *lab, int W, int H, int C, float *rgb);
+/* main.cpp:115-309 without file I/O and argument parsing: rgb (W*H*3, 0-255) is
+ * converted to Lab; depth (W*H, optional) is appended as channel 3 (IOUtility::cat,
+ * io_utility.cpp:453-470); the image is inpainted as by nli_inpaint; the result's colour
+ * channels are converted back into rgb_out (W*H*3) and, with depth, channel 3 goes to
+ * depth_out (W*H; IOUtility::get_depth, :506-513).  mask: W*H, 1 = hole (main.cpp:126-133
+ * sets it where the mask image is positive).  As main.cpp:166-203 does: params->coarse = 0
+ * selects min(1.5 * patch / max distance to the hole boundary, 1), a coarsest rate
+ * within 1e-5 of 1 runs one scale, and conf_t must be positive.  NLI_UPD_DEPTH needs
+ * depth (main.cpp:256-257). */
+int nli_inpaint_rgb(nli_ctx *ctx, const nli_inpaint_params *params, const float *rgb, const float *depth,
+                    const uint8_t *mask, int W, int H, float *rgb_out, float *depth_out,
+                    nli_inpaint_stats *stats);
+/* the same with device pointers (depth and depth_out may be NULL) */
+int nli_inpaint_rgb_dev(nli_ctx *ctx, const nli_inpaint_params *params, const float *rgb,
+                        const float *depth, const uint8_t *mask, int W, int H, float *rgb_out,
+                        float *depth_out, nli_inpaint_stats *stats);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,8 @@
  *   HipPatchNonLocalDepth         : AImageUpdating  (nli_update, two NL-Poisson solves)
  *   ImageInpainting               the multiscale driver of image_inpainting.cpp:68-557,
  *                                 with every numerical step a C-ABI call
+ *   IOUtility                     rgb_to_lab / lab_to_rgb on the GPU (nli_rgb_to_lab,
+ *                                 nli_lab_to_rgb), cat / separate / get_depth on the host
  *
  * Differences from the reference interface, all deliberate:
  *   - PatchMatch::calculate is virtual and PatchMatch is abstract: the GPU subclass is
@@ -239,6 +241,66 @@ public:
         Mask c;
         c.copy_from(*this);
         return c;
+    }
+};
+
+/* ---- IOUtility: colour conversion and RGB-D channels (io_utility.h:51-65) ----
+ * rgb_to_lab / lab_to_rgb run on the GPU (nli_rgb_to_lab / nli_lab_to_rgb) and take the
+ * Context every GPU object takes; images they do not convert come back as deep copies,
+ * as the reference returns its argument.  cat / separate / get_depth are the reference's
+ * host-side channel copies with its messages.  Image file I/O (iio) is not provided. */
+struct IOUtility {
+    static Image<float> rgb_to_lab(Context &ctx, FixedImage<float> image)   // io_utility.cpp:268-289
+    {
+        if (image.get_number_of_channels() != 3) return Image<float>(image);
+        Image<float> out(image.get_size(), 3u);
+        ctx.check(nli_rgb_to_lab(ctx.get(), image.raw(), (int)image.get_size_x(), (int)image.get_size_y(), 3,
+                                 out.raw()),
+                  "rgb_to_lab");
+        return out;
+    }
+    static Image<float> lab_to_rgb(Context &ctx, FixedImage<float> image)   // io_utility.cpp:292-318
+    {
+        const unsigned C = image.get_number_of_channels();
+        if (C != 3 && C != 4) return Image<float>(image);
+        Image<float> out(image.get_size(), 3u);
+        ctx.check(nli_lab_to_rgb(ctx.get(), image.raw(), (int)image.get_size_x(), (int)image.get_size_y(), (int)C,
+                                 out.raw()),
+                  "lab_to_rgb");
+        return out;
+    }
+    static Image<float> cat(FixedImage<float> image, FixedImage<float> depth)   // io_utility.cpp:453-470
+    {
+        // the reference throws only when both channel counts are wrong (`&&`, :455)
+        if (image.get_number_of_channels() != 3 && depth.get_number_of_channels() != 1)
+            throw std::runtime_error("cat: Channels do not match.");
+        if (image.get_size() != depth.get_size()) throw std::runtime_error("cat: Size does not match.");
+        Image<float> out(image.get_size(), 4u);
+        for (unsigned y = 0; y < image.get_size_y(); y++)
+            for (unsigned x = 0; x < image.get_size_x(); x++) {
+                for (unsigned c = 0; c < 3; c++) out(x, y, c) = image.at(x, y, c);
+                out(x, y, 3) = depth.at(x, y);
+            }
+        return out;
+    }
+    static void separate(FixedImage<float> input, Image<float> &rgb, Image<float> &depth)   // :491-504
+    {
+        if (input.get_number_of_channels() != 4) throw std::runtime_error("Separate: Channels do not match.");
+        if (rgb.get_size() != input.get_size() || rgb.get_number_of_channels() < 3 ||
+            depth.get_size() != input.get_size())
+            throw std::out_of_range("image index out of range");
+        for (unsigned y = 0; y < input.get_size_y(); y++)
+            for (unsigned x = 0; x < input.get_size_x(); x++) {
+                for (unsigned c = 0; c < 3; c++) rgb(x, y, c) = input(x, y, c);
+                depth(x, y) = input(x, y, 3);
+            }
+    }
+    static Image<float> get_depth(FixedImage<float> input)   // io_utility.cpp:506-513
+    {
+        if (input.get_number_of_channels() != 4) throw std::runtime_error("Separate: Channels do not match.");
+        Image<float> rgb(input.get_size(), 3u), depth(input.get_size());
+        separate(input, rgb, depth);
+        return depth;
     }
 };
 
@@ -718,7 +780,17 @@ private:
 extern "C" int nli_plain_inpaint(nli_ctx *ctx, const nli_inpaint_params *params, const float *in,
                                  const uint8_t *mask, int W, int H, int C, float *out,
                                  nli_inpaint_stats *stats);
-/* message of the last failed nli_plain_inpaint on this host thread */
+/* main.cpp:115-309 around that composition, without file I/O and argument parsing:
+ * rgb (W*H*3, 0-255) -> IOUtility::rgb_to_lab, the mask taken where positive
+ * (main.cpp:125-138), depth (W*H, optional) appended by IOUtility::cat, the parameter
+ * rules of main.cpp:166-203 (params->coarse = 0: the rate from the largest distance to
+ * the hole boundary; a rate within 1e-5 of 1: one scale), process, then
+ * IOUtility::lab_to_rgb into rgb_out (W*H*3) and IOUtility::get_depth into depth_out
+ * (W*H, with depth).  Exported by libnli_plugins.so; its results equal nli_inpaint_rgb's. */
+extern "C" int nli_plain_inpaint_rgb(nli_ctx *ctx, const nli_inpaint_params *params, const float *rgb,
+                                     const float *depth, const uint8_t *mask, int W, int H, float *rgb_out,
+                                     float *depth_out, nli_inpaint_stats *stats);
+/* message of the last failed nli_plain_inpaint(_rgb) on this host thread */
 extern "C" const char *nli_plain_last_error(void);
 
 #endif /* NLI_PLUGINS_HPP_ */

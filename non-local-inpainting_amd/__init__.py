@@ -45,11 +45,12 @@ EXPORTS = [
     "nli_patch_energy", "nli_energy_gradient", "nli_patch_extract", "nli_adjoint_aggregate",
     "nli_pm_tiles_per_block",
     "nli_profile_read", "nli_inpaint_batch", "nli_inpaint_batch_dev", "nli_inpaint_batch_each",
-    "nli_inpaint_batch_each_dev",
+    "nli_inpaint_batch_each_dev", "nli_rgb_to_lab", "nli_lab_to_rgb", "nli_rgb_to_lab_dev",
+    "nli_lab_to_rgb_dev", "nli_inpaint_rgb", "nli_inpaint_rgb_dev",
 ]
 # symbols of libnli_plugins.so (include/nli_plugins.hpp)
-PLUGIN_EXPORTS = ["nli_plain_inpaint", "nli_plain_last_error"]
-KERNELS = {"pm_sweep": 0, "pm_init": 1, "cg_iteration": 2, "poisson_coef": 3, "nl_gather": 4}
+PLUGIN_EXPORTS = ["nli_plain_inpaint", "nli_plain_inpaint_rgb", "nli_plain_last_error"]
+KERNELS = {"pm_sweep": 0, "pm_init": 1, "cg_iteration": 2, "poisson_coef": 3, "nl_gather": 4, "color": 6}
 
 
 class NliError(RuntimeError):
@@ -555,6 +556,72 @@ class Context:
                                          _ptr(out, _f32p), ct.byref(st)))
         return out, stats_of(st)
 
+
+    # ---- the colour path around process() (main.cpp:117, 146, 307-309) ----
+    def rgb_to_lab(self, img):
+        """IOUtility::rgb_to_lab (io_utility.cpp:268-289): 0-255 RGB to L*a*b*; images
+        that do not have 3 channels come back unchanged."""
+        img = _c(img, np.float32)
+        H, W, C = img.shape
+        out = np.zeros((H, W, C), np.float32)
+        self._check(self.lib.nli_rgb_to_lab(self.h, _ptr(img, _f32p), W, H, C, _ptr(out, _f32p)))
+        return out
+
+    def lab_to_rgb(self, img):
+        """IOUtility::lab_to_rgb (io_utility.cpp:292-318): L*a*b* (3 or 4 channels; the
+        fourth is not converted) to 0-255 RGB; other channel counts come back unchanged."""
+        img = _c(img, np.float32)
+        H, W, C = img.shape
+        out = np.zeros((H, W, 3 if C in (3, 4) else C), np.float32)
+        self._check(self.lib.nli_lab_to_rgb(self.h, _ptr(img, _f32p), W, H, C, _ptr(out, _f32p)))
+        return out
+
+    def rgb_to_lab_dev(self, d_in: int, d_out: int, W: int, H: int, C: int = 3):
+        self._check(self.lib.nli_rgb_to_lab_dev(self.h, ct.c_void_p(d_in), W, H, C, ct.c_void_p(d_out)))
+
+    def lab_to_rgb_dev(self, d_in: int, d_out: int, W: int, H: int, C: int = 3):
+        self._check(self.lib.nli_lab_to_rgb_dev(self.h, ct.c_void_p(d_in), W, H, C, ct.c_void_p(d_out)))
+
+    def inpaint_rgb(self, rgb, mask, params: Inpainting, depth=None):
+        """main.cpp:115-309 without the file I/O: 0-255 RGB (and an optional depth plane)
+        in, Lab inpainting, 0-255 RGB (and depth) out.  Returns (rgb, depth or None, stats)."""
+        rgb = _c(rgb, np.float32)
+        mask = _c(mask, np.uint8)
+        H, W, C = rgb.shape
+        if C != 3:
+            raise ValueError("rgb must be H x W x 3")
+        out = np.zeros_like(rgb)
+        d_in = _c(depth, np.float32) if depth is not None else None
+        d_out = np.zeros((H, W), np.float32) if depth is not None else None
+        p = params.to_struct()
+        st = InpaintStats()
+        self._check(self.lib.nli_inpaint_rgb(self.h, ct.byref(p), _ptr(rgb, _f32p), _ptr(d_in, _f32p),
+                                             _ptr(mask, _u8p), W, H, _ptr(out, _f32p), _ptr(d_out, _f32p),
+                                             ct.byref(st)))
+        return out, d_out, stats_of(st)
+
+    def inpaint_rgb_plain(self, rgb, mask, params: Inpainting, depth=None):
+        """main.cpp:115-309 through the C++ plugin classes (IOUtility, the plain drop-in);
+        returns (rgb, depth or None, stats) like inpaint_rgb."""
+        lib = load_plugins()
+        rgb = _c(rgb, np.float32)
+        mask = _c(mask, np.uint8)
+        H, W, _ = rgb.shape
+        out = np.zeros_like(rgb)
+        d_in = _c(depth, np.float32) if depth is not None else None
+        d_out = np.zeros((H, W), np.float32) if depth is not None else None
+        p = params.to_struct()
+        st = InpaintStats()
+        rc = lib.nli_plain_inpaint_rgb(self.h, ct.byref(p), _ptr(rgb, _f32p), _ptr(d_in, _f32p), _ptr(mask, _u8p),
+                                       W, H, _ptr(out, _f32p), _ptr(d_out, _f32p), ct.byref(st))
+        if rc != NLI_OK:
+            msg = lib.nli_plain_last_error().decode(errors="replace")
+            if rc == NLI_E_EMPTY_SOURCE:
+                raise EmptySourceError(rc, msg)
+            if rc == NLI_E_ARGUMENT:
+                raise ValueError(msg)
+            raise NliError(rc, msg)
+        return out, d_out, stats_of(st)
 
     def inpaint_plain(self, image, mask, params: Inpainting):
         """The plain drop-in: the reference's main() composition of the C++ plugin classes

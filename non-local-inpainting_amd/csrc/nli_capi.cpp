@@ -10,6 +10,7 @@
 #include <algorithm>
 
 #include <atomic>
+#include <cmath>
 #include <cfloat>
 #include <cstring>
 #include <memory>
@@ -45,6 +46,7 @@ struct nli_ctx {
     nli::DBuf<int32_t> n0, n1;
     nli::DBuf<int> pts, counts, total, nonempty;
     nli::DBuf<double> d0, d1;
+    nli::DBuf<float> c0, c1, c2, c3;   // colour path: Lab in/out, RGB, depth
     nli::AdjWork adj;
     std::vector<std::unique_ptr<nli_worker>> workers;   // batch workers, created on demand
 };
@@ -894,6 +896,141 @@ int nli_inpaint(nli_ctx *ctx, const nli_inpaint_params *params, const float *in,
         stage_out(ctx, out, (const float *)d_out, n * C);
         sync(ctx);
         report(rep, stats);
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+// one colour conversion on the context's stream.  IOUtility returns images it does not
+// convert unchanged (io_utility.cpp:270-272, 294-296); those are copied through.
+int color_out_channels(bool to_lab, int C) { return to_lab ? C : ((C == 3 || C == 4) ? 3 : C); }
+
+void color_dev(nli_ctx *ctx, bool to_lab, const float *in, int W, int H, int C, float *out)
+{
+    need(in && out && W > 0 && H > 0 && C >= 1, "bad colour conversion arguments");
+    const long n = (long)W * H;
+    if (to_lab && C == 3)
+        nli::rgb_to_lab(in, nullptr, n, 3, out, ctx->stream);
+    else if (!to_lab && (C == 3 || C == 4))
+        nli::lab_to_rgb(in, n, C, out, nullptr, ctx->stream);
+    else
+        NLI_HIP(hipMemcpyAsync(out, in, sizeof(float) * (size_t)n * C, hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+void color_host(nli_ctx *ctx, bool to_lab, const float *in, int W, int H, int C, float *out)
+{
+    need(in && out && W > 0 && H > 0 && C >= 1, "bad colour conversion arguments");
+    const size_t n = (size_t)W * H;
+    const float *d_in = stage_in(ctx, ctx->c0, in, n * C);
+    float *d_out = ctx->c1.ensure(n * color_out_channels(to_lab, C));
+    color_dev(ctx, to_lab, d_in, W, H, C, d_out);
+    stage_out(ctx, out, (const float *)d_out, n * color_out_channels(to_lab, C));
+    sync(ctx);
+}
+
+// main.cpp:115-309 around ImageInpainting::process, on device buffers: rgb (n x 3) and
+// depth (n, optional) -> Lab (+ depth) in lab_in -> inpainting into lab_out -> rgb_out
+// (n x 3) and depth_out (n, optional).  lab_in and lab_out hold n x (3 or 4) floats.
+void inpaint_rgb(nli_ctx *ctx, const nli_inpaint_params *params, const float *rgb, const float *depth,
+                 const uint8_t *mask, int W, int H, float *lab_in, float *lab_out, float *rgb_out,
+                 float *depth_out, nli_inpaint_stats *stats)
+{
+    nli::InpaintConfig cfg = inpaint_config(params);
+    need(rgb && mask && rgb_out && W > 0 && H > 0, "bad inpainting arguments");
+    need(cfg.method != NLI_UPD_DEPTH || depth, "Specify depth input using -depth [filename]");
+    const size_t n = (size_t)W * H;
+    if (cfg.coarse == 0) {
+        // main.cpp:166-178: the rule of thumb "patch side no smaller than twice the size of
+        // the coarsest inpainting domain", from the largest distance to the hole boundary
+        float *dt = ctx->f0.ensure(n);
+        float *g = ctx->f1.ensure(n);
+        int *st = ctx->pts.ensure(2 * n);
+        nli::distance_transform(mask, W, H, dt, g, st, ctx->stream);
+        std::vector<float> h(n);
+        NLI_HIP(hipMemcpyAsync(h.data(), dt, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+        sync(ctx);
+        float max_dist = 0;
+        for (size_t k = 0; k < n; k++) max_dist = (h[k] > max_dist) ? h[k] : max_dist;
+        cfg.coarse = std::min(1.5f * cfg.patch / max_dist, 1.f);
+    }
+    need(params->conf_t > 0.f, "ERROR: confidence_decay_time needs to be positive.");
+    if (std::abs(cfg.coarse - 1.0f) < 0.00001f) cfg.scales = 1;   // main.cpp:200-203
+    const int C = depth ? 4 : 3;
+    nli::rgb_to_lab(rgb, depth, (long)n, C, lab_in, ctx->stream);
+    nli::InpaintReport rep{};
+    nli::inpaint(cfg, lab_in, mask, W, H, C, lab_out, rep, ctx->drv, ctx->pm, ctx->upd, ctx->stream);
+    nli::lab_to_rgb(lab_out, (long)n, C, rgb_out, depth ? depth_out : nullptr, ctx->stream);
+    report(rep, stats);
+}
+
+}  // namespace
+
+extern "C" {
+
+int nli_rgb_to_lab(nli_ctx *ctx, const float *rgb, int W, int H, int C, float *lab)
+{
+    return guard(ctx, [&] { color_host(ctx, true, rgb, W, H, C, lab); });
+}
+
+int nli_lab_to_rgb(nli_ctx *ctx, const float *lab, int W, int H, int C, float *rgb)
+{
+    return guard(ctx, [&] { color_host(ctx, false, lab, W, H, C, rgb); });
+}
+
+int nli_rgb_to_lab_dev(nli_ctx *ctx, const float *rgb, int W, int H, int C, float *lab)
+{
+    return guard(ctx, [&] {
+        color_dev(ctx, true, rgb, W, H, C, lab);
+        sync(ctx);
+    });
+}
+
+int nli_lab_to_rgb_dev(nli_ctx *ctx, const float *lab, int W, int H, int C, float *rgb)
+{
+    return guard(ctx, [&] {
+        color_dev(ctx, false, lab, W, H, C, rgb);
+        sync(ctx);
+    });
+}
+
+int nli_inpaint_rgb(nli_ctx *ctx, const nli_inpaint_params *params, const float *rgb, const float *depth,
+                    const uint8_t *mask, int W, int H, float *rgb_out, float *depth_out, nli_inpaint_stats *stats)
+{
+    return guard(ctx, [&] {
+        need(rgb && mask && rgb_out && W > 0 && H > 0 && (!depth || depth_out), "bad inpainting arguments");
+        const size_t n = (size_t)W * H;
+        bool empty = true;
+        for (size_t k = 0; k < n && empty; k++) empty = mask[k] == 0;
+        need(!empty, "ERROR: Mask should not be empty (all black)");
+        const int C = depth ? 4 : 3;
+        const float *d_rgb = stage_in(ctx, ctx->c2, rgb, n * 3);
+        const float *d_depth = depth ? stage_in(ctx, ctx->c3, depth, n) : nullptr;
+        const uint8_t *d_m = stage_in(ctx, ctx->b0, mask, n);
+        float *lab_in = ctx->c0.ensure(n * C);
+        float *lab_out = ctx->c1.ensure(n * C);
+        // the RGB input and depth planes are read before the outputs are written
+        inpaint_rgb(ctx, params, d_rgb, d_depth, d_m, W, H, lab_in, lab_out, ctx->c2.p, depth ? ctx->c3.p : nullptr,
+                    stats);
+        stage_out(ctx, rgb_out, (const float *)ctx->c2.p, n * 3);
+        if (depth) stage_out(ctx, depth_out, (const float *)ctx->c3.p, n);
+        sync(ctx);
+    });
+}
+
+int nli_inpaint_rgb_dev(nli_ctx *ctx, const nli_inpaint_params *params, const float *rgb, const float *depth,
+                        const uint8_t *mask, int W, int H, float *rgb_out, float *depth_out,
+                        nli_inpaint_stats *stats)
+{
+    return guard(ctx, [&] {
+        need(rgb && mask && rgb_out && W > 0 && H > 0 && (!depth || depth_out), "bad inpainting arguments");
+        const size_t n = (size_t)W * H;
+        const int C = depth ? 4 : 3;
+        float *lab_in = ctx->c0.ensure(n * C);
+        float *lab_out = ctx->c1.ensure(n * C);
+        inpaint_rgb(ctx, params, rgb, depth, mask, W, H, lab_in, lab_out, rgb_out, depth_out, stats);
+        sync(ctx);
     });
 }
 

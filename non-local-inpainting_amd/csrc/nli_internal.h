@@ -90,7 +90,8 @@ enum TimedKernel {
     TK_POISSON_COEF = 3,
     TK_NL_GATHER = 4,   // NL-means / NL-medians gathers
     TK_PYRAMID = 5,
-    TK_COUNT = 6
+    TK_COLOR = 6,       // sRGB <-> Lab conversions
+    TK_COUNT = 7
 };
 namespace prof {
 // kernels: bitmask over TimedKernel ids (0 = timing off)
@@ -126,6 +127,13 @@ void fill_f64(double *p, double v, size_t n, hipStream_t s);
 
 // ---- gradient (k_patchmatch.hip) ----
 void gradient(const float *img, int W, int H, int C, float *grad, hipStream_t s);
+
+// ---- sRGB <-> Lab (k_color.hip) ----
+// rgb: n x 3 (0-255); lab: n x out_c, out_c = 3, or 4 with depth[n] as channel 3
+void rgb_to_lab(const float *rgb, const float *depth, long n, int out_c, float *lab, hipStream_t s);
+// lab: n x in_c (3 or 4; channel 3 is not converted); rgb: n x 3; depth (optional,
+// in_c = 4) receives channel 3
+void lab_to_rgb(const float *lab, long n, int in_c, float *rgb, float *depth, hipStream_t s);
 
 // ---- masks (k_masks.hip) ----
 void extended_domain(const uint8_t *hole, int W, int H, int P, uint8_t *ext, uint8_t *scratch,

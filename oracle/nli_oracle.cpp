@@ -552,6 +552,76 @@ void nlo_gaussian_weights(int size_x, int size_y, float sigma_x, float sigma_y, 
     for (unsigned i = 0; i < sx * sy; i++) values[i] /= sum;
 }
 
+/* ---- colour conversion (io_utility.cpp:268-318, per-pixel steps :380-450) ----
+ * Each step keeps the reference's types: float arithmetic, except where a double
+ * constant promotes an expression (the 0.04045 test and the /12.92 branch of the
+ * gamma expansion, the a/500.0 term of the inverse), and powf for pow(float, float). */
+static void lab_of_rgb(const float *in, float *out)
+{
+    float lin[3];
+    for (int c = 0; c < 3; c++) {
+        const float v = in[c] / 255.0f;                                 /* :382-384 */
+        const float e = ((double)v > 0.04045) ? powf((v + 0.055f) / 1.055f, 2.4f)
+                                               : (float)((double)v / 12.92);   /* :386-388 */
+        lin[c] = e * 100.0f;                                            /* :390-392 */
+    }
+    /* :394-396, sRGB primaries to XYZ (D65); sums left to right */
+    const float X = lin[0] * 0.412453f + lin[1] * 0.357580f + lin[2] * 0.180423f;
+    const float Y = lin[0] * 0.212671f + lin[1] * 0.715160f + lin[2] * 0.072169f;
+    const float Z = lin[0] * 0.019334f + lin[1] * 0.119193f + lin[2] * 0.950227f;
+    /* :403-409, white point normalisation and the cube-root / linear toe */
+    float f[3] = {X / 95.047f, Y / 100.000f, Z / 108.883f};
+    for (int c = 0; c < 3; c++)
+        f[c] = (f[c] > 0.008856f) ? powf(f[c], 1.0f / 3.0f) : (7.787f * f[c]) + (16.0f / 116.0f);
+    out[0] = (116.0f * f[1]) - 16.0f;                                   /* :411-413 */
+    out[1] = 500.0f * (f[0] - f[1]);
+    out[2] = 200.0f * (f[1] - f[2]);
+}
+
+static void rgb_of_lab(const float *in, float *out)
+{
+    float f[3];
+    f[1] = (in[0] + 16.0f) / 116.0f;                                    /* :419-421 */
+    f[0] = (float)((double)in[1] / 500.0 + (double)f[1]);
+    f[2] = f[1] - in[2] / 200.0f;
+    const float white[3] = {95.047f, 100.000f, 108.883f};
+    float xyz[3];
+    for (int c = 0; c < 3; c++) {                                       /* :423-429 */
+        const float cube = powf(f[c], 3.0f);
+        const float t = (cube > 0.008856f) ? cube : (f[c] - 16.0f / 116.0f) / 7.787f;
+        xyz[c] = t * white[c];
+    }
+    const float x = xyz[0] / 100.0f, y = xyz[1] / 100.0f, z = xyz[2] / 100.0f;   /* :435-437 */
+    float lin[3];                                                       /* :439-441 */
+    lin[0] = x * 3.240479f + y * -1.537150f + z * -0.498535f;
+    lin[1] = x * -0.969256f + y * 1.875992f + z * 0.041556f;
+    lin[2] = x * 0.055648f + y * -0.204043f + z * 1.057311f;
+    for (int c = 0; c < 3; c++) {                                       /* :443-449 */
+        const float g = (lin[c] > 0.0031308f) ? 1.055f * powf(lin[c], 1.0f / 2.4f) - 0.055f : 12.92f * lin[c];
+        out[c] = g * 255.0f;
+    }
+}
+
+void nlo_rgb_to_lab(const float *rgb, int W, int H, int C, float *lab)
+{
+    const size_t n = (size_t)W * H;
+    if (C != 3) {                                                       /* :270-272 */
+        std::memcpy(lab, rgb, sizeof(float) * n * C);
+        return;
+    }
+    for (size_t i = 0; i < n; i++) lab_of_rgb(rgb + 3 * i, lab + 3 * i);
+}
+
+void nlo_lab_to_rgb(const float *lab, int W, int H, int C, float *rgb)
+{
+    const size_t n = (size_t)W * H;
+    if (C != 3 && C != 4) {                                             /* :294-296 */
+        std::memcpy(rgb, lab, sizeof(float) * n * C);
+        return;
+    }
+    for (size_t i = 0; i < n; i++) rgb_of_lab(lab + (size_t)C * i, rgb + 3 * i);
+}
+
 /* gradient.cpp:20-76, forward differences in x and y, zero on the last column/row */
 void nlo_gradient(const float *img, int W, int H, int C, float *grad)
 {

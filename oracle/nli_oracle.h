@@ -107,6 +107,12 @@ int   nlo_downsample(const float *img, int W, int H, int C, float factor, float 
 int   nlo_downsample_mask(const uint8_t *mask, int W, int H, float factor, float threshold,
                           uint8_t *out);
 void  nlo_upsample(const float *in, int W, int H, int C, int W2, int H2, float *out);
+/* sRGB (0-255) <-> CIE L*a*b*, io_utility.cpp:268-318 and :380-450.  Images of other
+ * channel counts are copied through (rgb_to_lab: C != 3; lab_to_rgb: C not 3 or 4);
+ * lab_to_rgb of a 4-channel image converts channels 0-2.  out holds W*H*3 floats when
+ * converting, W*H*C otherwise. */
+void  nlo_rgb_to_lab(const float *rgb, int W, int H, int C, float *lab);
+void  nlo_lab_to_rgb(const float *lab, int W, int H, int C, float *rgb);
 
 /* ---- per-scale masks ---- */
 void  nlo_extended_domain(const uint8_t *hole, int W, int H, int P, uint8_t *out);

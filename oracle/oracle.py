@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libnli_ref.so")
 REF_OMP_SO = os.path.join(HERE, "_ref", "libnli_ref_omp.so")
+REF_COLOR_SO = os.path.join(HERE, "_ref", "libnli_ref_color.so")
 
 DIST = {"l2": 0, "l1": 1, "l2combined": 2, "l2rgbd": 3}
 UPD = {"nlmeans": 0, "nlmedians": 1, "nlpoisson": 2, "nldepth": 3}
@@ -195,6 +196,21 @@ class Oracle:
         H, W, C = img.shape
         out = np.zeros((H2, W2, C), np.float32)
         self.lib.nlo_upsample(_p(img, _f32p), W, H, C, W2, H2, _p(out, _f32p))
+        return out
+
+    # ---- colour (io_utility.cpp:268-318) ----
+    def rgb_to_lab(self, img):
+        img = _c(img, np.float32)
+        H, W, C = img.shape
+        out = np.zeros((H, W, C), np.float32)
+        self.lib.nlo_rgb_to_lab(_p(img, _f32p), W, H, C, _p(out, _f32p))
+        return out
+
+    def lab_to_rgb(self, img):
+        img = _c(img, np.float32)
+        H, W, C = img.shape
+        out = np.zeros((H, W, 3 if C in (3, 4) else C), np.float32)
+        self.lib.nlo_lab_to_rgb(_p(img, _f32p), W, H, C, _p(out, _f32p))
         return out
 
     # ---- masks ----
@@ -501,6 +517,31 @@ class Reference:
         if rc != 0:
             raise ValueError("ref_process failed")
         return out
+
+
+class ReferenceColor:
+    """The reference's IOUtility::rgb_to_lab / lab_to_rgb (io_utility.cpp:268-318),
+    compiled in place (oracle/_ref/libnli_ref_color.so); 3-channel images only."""
+
+    def __init__(self, path: str = REF_COLOR_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle ref)")
+        self.lib = ct.CDLL(path, mode=ct.RTLD_LOCAL)
+
+    def _run(self, fn, img):
+        img = _c(img, np.float32)
+        H, W, C = img.shape
+        if C != 3:
+            raise ValueError("3-channel images only")
+        out = np.zeros_like(img)
+        fn(_p(img, _f32p), W, H, _p(out, _f32p))
+        return out
+
+    def rgb_to_lab(self, img):
+        return self._run(self.lib.ref_rgb_to_lab, img)
+
+    def lab_to_rgb(self, img):
+        return self._run(self.lib.ref_lab_to_rgb, img)
 
 
 def nnf_energy(oracle: Oracle, dp: DistParams, img, nnf, tmask) -> float:
