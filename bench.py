@@ -303,9 +303,18 @@ def main():
     compare = None
     if not args.no_compare:
         # the same image alone on one stream (device-resident), and in FP64
+        ctx.profile(["pm_sweep"])
         t1 = time.perf_counter()
         st1 = ctx.inpaint_dev(d_in[0], d_mask[0], d_out[0], W, H, C, params)
         single = W * H / 1e6 / (time.perf_counter() - t1)
+        sw1 = ctx.profile_read("pm_sweep")
+        ctx.profile(False)
+        # the dominant kernel alone on the GPU (one driver): the figure a kernel trace of one
+        # image reports; the timed steps' events include the other drivers' concurrent work
+        a1 = sw1["work"] / sw1["ms"] / 1e9
+        roof["isolated"] = dict(achieved=a1, frac=a1 / FP32_PEAK_TFLOPS, launches=sw1["launches"],
+                                avg_launch_ms=sw1["ms"] / max(sw1["launches"], 1),
+                                run="one image on one driver, same parameters, outside the timed steps")
         out1 = ctx.from_device(d_out[0], wls[0].image.shape, np.float32)
         t1 = time.perf_counter()
         ctx.inpaint_dev(d_in[0], d_mask[0], d_out[0], W, H, C, workload_params(nli, nli.CG_FP64))
