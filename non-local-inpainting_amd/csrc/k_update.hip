@@ -1014,7 +1014,7 @@ CGArgs<T> cg_args(const CGParams &cg, int n, int nc, UpdWork &w, const Stencil *
     a.hist = w.scalars.p + kSummary;
     a.done_at = w.flag.p + 2;
     a.poll_host = w.done_dev;
-    a.poll_every = 8;
+    a.poll_every = 4;   // measured: 4 beats 8 (fewer iterations run past convergence) and 1-2
     a.tol = cg.tol;
     a.limit = cg.limit;
     a.refine_every = cg.refine_every;
