@@ -27,6 +27,9 @@
 //   from an FP64 evaluation of b - A x every `refine_every` iterations.
 #include "nli_internal.h"
 
+#include <atomic>
+#include <chrono>
+#include <cstring>
 #include <cfloat>
 #include <cmath>
 #include <type_traits>
@@ -37,6 +40,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxNC = 4;
+constexpr int kPollSlot = kMaxNC + 1;   // a CG poll slot: kMaxNC stop flags, then its stamp
 constexpr int kMaxP = 15;
 
 // scalar slots per channel
@@ -674,8 +678,9 @@ struct CGArgs {
     double *prr, *prq;
     double *hist;
     int *done_at;
-    int *poll_host;    // pinned host slots [2][kMaxNC]: stop state after every poll_every iterations
+    int *poll_host;    // pinned host slots [2][kPollSlot]: stop state after every poll_every iterations
     int poll_every;
+    unsigned poll_base;   // stamp of this solve's first poll (chunk k is announced as poll_base + k)
     float tol;
     int limit;
     int refine_every;  // T = float: residual recomputed in FP64 every k iterations (0 = never)
@@ -816,9 +821,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_a(CGArgs<T> a, int it)
     // the stop state for the host's convergence poll, written straight to pinned host
     // memory (no copy in the stream); slot parity follows the poll count
     if ((it + 1) % a.poll_every == 0 && blockIdx.x == 0 && threadIdx.x == 0 && a.poll_host) {
-        int *slot = a.poll_host + (((it + 1) / a.poll_every - 1) & 1) * kMaxNC;
+        const int chunk = (it + 1) / a.poll_every - 1;
+        int *slot = a.poll_host + (chunk & 1) * kPollSlot;
 #pragma unroll
         for (int c = 0; c < NC; c++) slot[c] = run[c] ? -1 : (done[c] >= 0 ? done[c] : it);
+        __threadfence_system();   // the flags reach host memory before the stamp that announces them
+        reinterpret_cast<volatile unsigned *>(slot)[kMaxNC] = a.poll_base + (unsigned)chunk;
     }
     if (!any) return;   // uniform over the grid
     double acc[NC];
@@ -955,6 +963,26 @@ void upload_weights(UpdWork &w, int P, float sigma, hipStream_t s)
     w.w_sigma = sigma;
 }
 
+// Waits until the poll slot holds the given stamp (written by k_cg_a after the flags)
+// and returns the slot.  If the stamp has not come after two seconds, the stream is
+// drained and the stamp must be there then.
+inline const volatile int *wait_poll(UpdWork &w, int slot_index, unsigned stamp, hipStream_t s)
+{
+    const volatile int *slot = w.done_host + slot_index * kPollSlot;
+    const volatile unsigned *st = reinterpret_cast<const volatile unsigned *>(slot + kMaxNC);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spins = 1; *st != stamp; spins++) {
+        __builtin_ia32_pause();
+        if ((spins & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+            NLI_HIP(hipStreamSynchronize(s));
+            if (*st != stamp) throw std::runtime_error("CG poll: the stop flags were not written");
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return slot;
+}
+
 // Runs the CG on n unknowns x NC channels; returns with x in a.x[done_at & 1].
 template <int NC, typename T>
 void run_cg(const CGParams &cg, int n, UpdWork &w, CGArgs<T> a, hipStream_t s)
@@ -962,12 +990,15 @@ void run_cg(const CGParams &cg, int n, UpdWork &w, CGArgs<T> a, hipStream_t s)
     const int nb = (int)grid_for(n);
     k_cg_init<NC, T><<<nb, kBlock, 0, s>>>(a, w.x0.p);
     NLI_LAUNCH_CHECK();
-    // Convergence is polled every a.poll_every iterations without draining the stream:
-    // after each chunk k_cg_a has written the stop flags to pinned host memory, an event
-    // marks the chunk, and the host waits for the previous chunk's event while the
-    // current chunk runs (kernels of a converged channel return at once).
+    // Convergence is polled every a.poll_every iterations without draining the stream
+    // and without markers in it: after each chunk, k_cg_a writes the stop flags and then
+    // the chunk's stamp to pinned host memory, and the host waits for the previous
+    // chunk's stamp while the current chunk runs (kernels of a converged channel return
+    // at once).
     if (!a.poll_host || a.poll_every < 1) throw std::logic_error("CG poll slots not set");
     const int poll = a.poll_every;
+    a.poll_base = w.poll_seq;
+    w.poll_seq += (unsigned)((cg.limit + 2) / poll + 2);
     // algorithmic bytes of one iteration of the reference recurrence (two matvecs):
     // 2 x 13 words per unknown and channel
     const double it_bytes = 2.0 * 13 * sizeof(T) * NC * (double)n;
@@ -979,13 +1010,12 @@ void run_cg(const CGParams &cg, int n, UpdWork &w, CGArgs<T> a, hipStream_t s)
         NLI_LAUNCH_CHECK();
         prof::end(TK_CG, s, it_bytes, (double)NC * n);
         if ((it + 1) % poll == 0 && it <= cg.limit) {
-            // k_cg_a of this iteration wrote the stop state to slot chunk & 1
-            NLI_HIP(hipEventRecord(w.poll_ev[chunk & 1], s));
+            // k_cg_a of this iteration writes the stop state to slot chunk & 1
             if (chunk > 0) {
                 const int prev = (chunk - 1) & 1;
-                NLI_HIP(hipEventSynchronize(w.poll_ev[prev]));
+                const volatile int *slot = wait_poll(w, prev, a.poll_base + (unsigned)(chunk - 1), s);
                 bool all = true;
-                for (int c = 0; c < NC; c++) all = all && w.done_host[prev * kMaxNC + c] >= 0;
+                for (int c = 0; c < NC; c++) all = all && slot[c] >= 0;
                 if (all) break;
             }
             chunk++;
@@ -1014,7 +1044,7 @@ CGArgs<T> cg_args(const CGParams &cg, int n, int nc, UpdWork &w, const Stencil *
     a.hist = w.scalars.p + kSummary;
     a.done_at = w.flag.p + 2;
     a.poll_host = w.done_dev;
-    a.poll_every = 4;   // measured: 4 beats 8 (fewer iterations run past convergence) and 1-2
+    a.poll_every = 2;   // measured with stamped polls: 2 (or 1) beats 4 and 8: fewer iterations run past convergence
     a.tol = cg.tol;
     a.limit = cg.limit;
     a.refine_every = cg.refine_every;
@@ -1036,10 +1066,10 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     const int nb = (int)grid_for(std::max(n, 1));
     w.scalars.ensure(kSummary + (size_t)kMaxNC * (cg.limit + 2));
     if (!w.done_host) {   // CG poll slots: coherent pinned memory the kernels write directly
-        NLI_HIP(hipHostMalloc((void **)&w.done_host, sizeof(int) * 2 * kMaxNC,
+        NLI_HIP(hipHostMalloc((void **)&w.done_host, sizeof(int) * 2 * kPollSlot,
                               hipHostMallocCoherent | hipHostMallocMapped));
+        std::memset(w.done_host, 0, sizeof(int) * 2 * kPollSlot);   // stamp 0: no poll yet
         NLI_HIP(hipHostGetDevicePointer((void **)&w.done_dev, w.done_host, 0));
-        for (hipEvent_t &e : w.poll_ev) NLI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     w.partials.ensure((size_t)2 * kMaxNC * nb + 8);
     w.flag.ensure(2 + kMaxNC + 2);

@@ -209,17 +209,15 @@ void patch_distance_field(const DistDesc &d, const float *src, int sW, int sH, c
 struct UpdWork {
     int w_patch = 0;          // cached patch weights
     float w_sigma = 0.0f;
-    int *done_host = nullptr; // pinned: pipelined CG convergence polls
+    int *done_host = nullptr; // pinned: pipelined CG convergence polls (flags + stamp per slot)
     int *done_dev = nullptr;  // the same memory as the kernels address it
-    hipEvent_t poll_ev[2] = {nullptr, nullptr};
+    unsigned poll_seq = 1;    // next unused poll stamp (stamps are never reused)
     UpdWork() = default;
     UpdWork(const UpdWork &) = delete;
     UpdWork &operator=(const UpdWork &) = delete;
     ~UpdWork()
     {
         if (done_host) (void)hipHostFree(done_host);
-        for (hipEvent_t e : poll_ev)
-            if (e) (void)hipEventDestroy(e);
     }
     DBuf<float> weights, grad, conf_one;
     DBuf<double> a1, a2, f1, f2, F1, F2, partials, scalars;
