@@ -301,6 +301,10 @@ int nli_inpaint_dev(nli_ctx *ctx, const nli_inpaint_params *params, const float 
 #define NLI_KERNEL_COLOR 6       /* sRGB <-> Lab conversions; work = bytes */
 #define NLI_PROFILE_ALL (-1)
 int nli_profile_enable(nli_ctx *ctx, int kernels);
+/* as nli_profile_enable, timing one launch in `period` of each selected kernel (counted
+ * in issue order over all streams), so that throughput runs carry few event pairs;
+ * nli_profile_read then reports the sampled launches only */
+int nli_profile_enable_sampled(nli_ctx *ctx, int kernels, int period);
 int nli_profile_read(nli_ctx *ctx, int kernel, int *launches, double *total_ms, double *work,
                      double *units);
 

@@ -46,7 +46,7 @@ EXPORTS = [
     "nli_pm_tiles_per_block",
     "nli_profile_read", "nli_inpaint_batch", "nli_inpaint_batch_dev", "nli_inpaint_batch_each",
     "nli_inpaint_batch_each_dev", "nli_rgb_to_lab", "nli_lab_to_rgb", "nli_rgb_to_lab_dev",
-    "nli_lab_to_rgb_dev", "nli_inpaint_rgb", "nli_inpaint_rgb_dev",
+    "nli_lab_to_rgb_dev", "nli_inpaint_rgb", "nli_inpaint_rgb_dev", "nli_profile_enable_sampled",
 ]
 # symbols of libnli_plugins.so (include/nli_plugins.hpp)
 PLUGIN_EXPORTS = ["nli_plain_inpaint", "nli_plain_inpaint_rgb", "nli_plain_last_error"]
@@ -378,16 +378,17 @@ class Context:
         return (img, total.value, st) if stats else (img, total.value)
 
     # ---- kernel timing ----
-    def profile(self, on=True):
+    def profile(self, on=True, period=1):
         """Time kernels with HIP events: True = all, False = none, or an iterable of
-        kernel names (keys of KERNELS).  Resets the totals."""
+        kernel names (keys of KERNELS); one launch in `period` of each is timed.  Resets
+        the totals."""
         if on is True or on is False:
             mask = -1 if on else 0
         else:
             mask = 0
             for k in on:
                 mask |= 1 << KERNELS[k]
-        self._check(self.lib.nli_profile_enable(self.h, mask))
+        self._check(self.lib.nli_profile_enable_sampled(self.h, mask, int(period)))
 
     def profile_read(self, kernel: str) -> dict:
         n = ct.c_int()

@@ -420,6 +420,15 @@ int nli_profile_enable(nli_ctx *ctx, int kernels)
     });
 }
 
+int nli_profile_enable_sampled(nli_ctx *ctx, int kernels, int period)
+{
+    return guard(ctx, [&] {
+        need(period >= 1, "sampling period must be at least 1");
+        sync(ctx);
+        nli::prof::enable((unsigned)kernels, (unsigned)period);
+    });
+}
+
 int nli_profile_read(nli_ctx *ctx, int kernel, int *launches, double *total_ms, double *work, double *units)
 {
     return guard(ctx, [&] {

@@ -94,8 +94,9 @@ enum TimedKernel {
     TK_COUNT = 7
 };
 namespace prof {
-// kernels: bitmask over TimedKernel ids (0 = timing off)
-void enable(unsigned kernels);
+// kernels: bitmask over TimedKernel ids (0 = timing off); one launch in `period` of each
+// kernel is timed (launches counted in issue order over all streams)
+void enable(unsigned kernels, unsigned period = 1);
 // brackets launches on stream s; `work` is the algorithmic flops or bytes of the
 // bracketed launches, `units` the elements they process
 void begin(int kernel, hipStream_t s);

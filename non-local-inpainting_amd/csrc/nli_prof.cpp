@@ -2,6 +2,7 @@
 // stream the kernels are launched on (the event pair brackets the launches).
 #include "nli_internal.h"
 
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -19,6 +20,8 @@ struct Rec {
 struct State {
     std::mutex mu;
     unsigned mask = 0;   // bit k: kernel k is timed
+    std::atomic<unsigned> period{1};     // time one launch in `period` of each kernel
+    std::atomic<unsigned long long> seen[TK_COUNT] = {};
     std::vector<Rec> open;           // begun, not ended (per kernel at most one)
     std::vector<Rec> done;
     std::vector<hipEvent_t> pool;
@@ -62,22 +65,30 @@ void drain(State &s)
 
 }  // namespace
 
-void enable(unsigned kernels)
+void enable(unsigned kernels, unsigned period)
 {
     State &s = st();
     std::lock_guard<std::mutex> g(s.mu);
     drain(s);
     s.mask = kernels & ((1u << TK_COUNT) - 1);
+    s.period = period > 0 ? period : 1;
     for (int k = 0; k < TK_COUNT; k++) {
         s.launches[k] = 0;
         s.ms[k] = s.work[k] = s.units[k] = 0;
+        s.seen[k] = 0;
     }
 }
+
+// whether this thread's last begin() of a kernel was sampled (begin and end of a launch
+// run on the thread that launches it); unsampled launches take no lock
+thread_local bool t_sampled[TK_COUNT] = {};
 
 void begin(int kernel, hipStream_t stream)
 {
     State &s = st();
     if (!((s.mask >> kernel) & 1u)) return;
+    t_sampled[kernel] = s.seen[kernel].fetch_add(1, std::memory_order_relaxed) % s.period.load() == 0;
+    if (!t_sampled[kernel]) return;
     std::lock_guard<std::mutex> g(s.mu);
     Rec r{kernel, stream, take(s), take(s), 0, 0};
     NLI_HIP(hipEventRecord(r.a, stream));
@@ -87,7 +98,8 @@ void begin(int kernel, hipStream_t stream)
 void end(int kernel, hipStream_t stream, double work, double units)
 {
     State &s = st();
-    if (!((s.mask >> kernel) & 1u)) return;
+    if (!((s.mask >> kernel) & 1u) || !t_sampled[kernel]) return;
+    t_sampled[kernel] = false;
     std::lock_guard<std::mutex> g(s.mu);
     for (size_t i = 0; i < s.open.size(); i++)
         if (s.open[i].kernel == kernel && s.open[i].stream == stream) {
