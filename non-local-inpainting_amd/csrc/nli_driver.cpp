@@ -151,16 +151,20 @@ void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask,
         return (int)nev++;
     };
     uint64_t pm_calls = 0;
+    // `start`: an event recorded just before (the end of the previous span, with nothing
+    // enqueued since), shared as this span's start so that a span costs one marker
     auto run_pm = [&](const float *img, const uint8_t *sm, const uint8_t *tm, int w, int h, const int32_t *init,
-                      int32_t *out) {
-        const int e0 = event();
+                      int32_t *out, int start = -1) {
+        const int e0 = start >= 0 ? start : event();
         patchmatch(dd, pmc, img, sm, w, h, img, tm, w, h, C, init, cfg.seed * 0x9E3779B97F4A7C15ull + (pm_calls++),
                    out, nullptr, pmw, s);
         spans.push_back({e0, event(), true});
+        return spans.back().b;
     };
     auto run_update = [&](float *img, const float *orig, const uint8_t *hole, const int *pts, int npts,
-                          const uint8_t *ext, const int32_t *nnf, const float *conf, int w, int h) {
-        const int e0 = event();
+                          const uint8_t *ext, const int32_t *nnf, const float *conf, int w, int h,
+                          int start = -1) {
+        const int e0 = start >= 0 ? start : event();
         double total = 0;
         CGStats st{};
         if (cfg.method == UPD_DEPTH) {
@@ -241,11 +245,13 @@ void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask,
         NLI_HIP(hipMemcpyAsync(dw.orig.p, img, n * C * sizeof(float), hipMemcpyDeviceToDevice, s));
         double total = std::numeric_limits<double>::max();
         int it = 0;
+        int mark = -1;   // end event of the previous span of this loop
         for (it = 0; it < cfg.iters && total > (double)cfg.tol; it++) {
             int32_t *out = next_buf(nnf_cur, nullptr);
-            run_pm(img, dw.src.p, dw.tgt.p, w, h, nnf_cur, out);
+            mark = run_pm(img, dw.src.p, dw.tgt.p, w, h, nnf_cur, out, mark);
             nnf_cur = out;
-            total = run_update(img, dw.orig.p, hole, dw.hole_pts.p, nh, dw.ext.p, nnf_cur, dw.conf.p, w, h);
+            total = run_update(img, dw.orig.p, hole, dw.hole_pts.p, nh, dw.ext.p, nnf_cur, dw.conf.p, w, h, mark);
+            mark = spans.back().b;
         }
         rep.iterations[L] = it;
         rep.last_total[L] = total;
