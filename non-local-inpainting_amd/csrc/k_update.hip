@@ -95,9 +95,13 @@ __device__ inline void block_reduce_store(double (&v)[NC], double *partials, int
 // after the kernel that stored the partials, so no grid-wide fence is needed.  With a
 // CG history (hist != nullptr) it also writes the CG part of the update's summary: per
 // channel the stopping iteration, |r_0|^2 and the final |r|^2.
+// The summary also goes to pinned host memory (host_out, when given), followed by a
+// stamp that the host waits for instead of a copy and a stream synchronisation: each
+// writer fences at system scope before the barrier after which thread 0 writes the stamp.
 __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restrict__ partials, int nb,
                                                          double *__restrict__ out, const double *__restrict__ hist,
-                                                         int hist_len, const int *__restrict__ done_at, int nc)
+                                                         int hist_len, const int *__restrict__ done_at, int nc,
+                                                         double *host_out, unsigned *host_stamp, unsigned stamp)
 {
     __shared__ double sh[kBlock];
     double s = 0.0;
@@ -108,13 +112,27 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restric
         if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[kSumTotal] = sh[0];
+    if (threadIdx.x == 0) {
+        out[kSumTotal] = sh[0];
+        if (host_out) host_out[kSumTotal] = sh[0];
+    }
     const int c = threadIdx.x;
     if (hist && c < nc) {
         const int d = done_at[c];
-        out[kSumDone + c] = (double)d;
-        out[kSumNr0 + c] = hist[(size_t)c * hist_len];
-        out[kSumNr + c] = hist[(size_t)c * hist_len + (d > 0 ? d : 0)];
+        const double v[3] = {(double)d, hist[(size_t)c * hist_len], hist[(size_t)c * hist_len + (d > 0 ? d : 0)]};
+        out[kSumDone + c] = v[0];
+        out[kSumNr0 + c] = v[1];
+        out[kSumNr + c] = v[2];
+        if (host_out) {
+            host_out[kSumDone + c] = v[0];
+            host_out[kSumNr0 + c] = v[1];
+            host_out[kSumNr + c] = v[2];
+        }
+    }
+    if (host_out) {
+        __threadfence_system();
+        __syncthreads();
+        if (threadIdx.x == 0) *reinterpret_cast<volatile unsigned *>(host_stamp) = stamp;
     }
 }
 
@@ -943,14 +961,6 @@ __global__ void k_poisson_write(float *__restrict__ img, const float *__restrict
 
 inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-double read_scalar(const double *d, hipStream_t s)
-{
-    double v = 0;
-    NLI_HIP(hipMemcpyAsync(&v, d, sizeof(double), hipMemcpyDeviceToHost, s));
-    NLI_HIP(hipStreamSynchronize(s));
-    return v;
-}
-
 void upload_weights(UpdWork &w, int P, float sigma, hipStream_t s)
 {
     if (w.w_patch == P && w.w_sigma == sigma) return;
@@ -963,24 +973,52 @@ void upload_weights(UpdWork &w, int P, float sigma, hipStream_t s)
     w.w_sigma = sigma;
 }
 
-// Waits until the poll slot holds the given stamp (written by k_cg_a after the flags)
-// and returns the slot.  If the stamp has not come after two seconds, the stream is
-// drained and the stamp must be there then.
-inline const volatile int *wait_poll(UpdWork &w, int slot_index, unsigned stamp, hipStream_t s)
+// Waits until *st (pinned host memory a kernel writes after its data) holds `stamp`.  If
+// it has not come after two seconds, the stream is drained and it must be there then.
+inline void wait_stamp(const volatile unsigned *st, unsigned stamp, hipStream_t s, const char *what)
 {
-    const volatile int *slot = w.done_host + slot_index * kPollSlot;
-    const volatile unsigned *st = reinterpret_cast<const volatile unsigned *>(slot + kMaxNC);
     const auto t0 = std::chrono::steady_clock::now();
     for (unsigned spins = 1; *st != stamp; spins++) {
         __builtin_ia32_pause();
         if ((spins & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
             NLI_HIP(hipStreamSynchronize(s));
-            if (*st != stamp) throw std::runtime_error("CG poll: the stop flags were not written");
+            if (*st != stamp) throw std::runtime_error(std::string(what) + ": the device did not write its result");
             break;
         }
     }
     std::atomic_thread_fence(std::memory_order_acquire);
+}
+
+// Waits until the poll slot holds the given stamp (written by k_cg_a after the flags)
+// and returns the slot.
+inline const volatile int *wait_poll(UpdWork &w, int slot_index, unsigned stamp, hipStream_t s)
+{
+    const volatile int *slot = w.done_host + slot_index * kPollSlot;
+    wait_stamp(reinterpret_cast<const volatile unsigned *>(slot + kMaxNC), stamp, s, "CG poll");
     return slot;
+}
+
+// The update's summary (k_sum_partials' scalars) from pinned memory: the kernel was
+// launched with sum_args(w) and stamp w.sum_seq.
+struct SumArgs {
+    double *host_out;
+    unsigned *host_stamp;
+    unsigned stamp;
+};
+inline SumArgs sum_args(UpdWork &w)
+{
+    if (!w.sum_host) {
+        NLI_HIP(hipHostMalloc((void **)&w.sum_host, sizeof(double) * (kSummary + 1),
+                              hipHostMallocCoherent | hipHostMallocMapped));
+        std::memset(w.sum_host, 0, sizeof(double) * (kSummary + 1));   // stamp 0: nothing written yet
+        NLI_HIP(hipHostGetDevicePointer((void **)&w.sum_dev, w.sum_host, 0));
+    }
+    return SumArgs{w.sum_dev, reinterpret_cast<unsigned *>(w.sum_dev + kSummary), ++w.sum_seq};
+}
+inline const volatile double *wait_summary(UpdWork &w, const SumArgs &a, hipStream_t s)
+{
+    wait_stamp(reinterpret_cast<const volatile unsigned *>(w.sum_host + kSummary), a.stamp, s, "update summary");
+    return w.sum_host;
 }
 
 // Runs the CG on n unknowns x NC channels; returns with x in a.x[done_at & 1].
@@ -1073,7 +1111,6 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     }
     w.partials.ensure((size_t)2 * kMaxNC * nb + 8);
     w.flag.ensure(2 + kMaxNC + 2);
-    double *result = w.scalars.p + kSumTotal;
     if (n == 0) {
         if (stats)
             for (int c = 0; c < nc; c++) { stats->iterations[c] = 0; stats->nr0[c] = 0; stats->nr[c] = 0; }
@@ -1146,12 +1183,11 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
         k_poisson_write<float><<<nb, kBlock, 0, s>>>(image, orig, Cs, c0, nc, hole_pts, n, w.xs0.p, w.xs1.p, done_at,
                                                      w.partials.p);
     }
+    const SumArgs sa = sum_args(w);
     k_sum_partials<<<1, kBlock, 0, s>>>(w.partials.p, nb, w.scalars.p, w.scalars.p + kSummary, cg.limit + 2,
-                                        done_at, nc);
+                                        done_at, nc, sa.host_out, sa.host_stamp, sa.stamp);
     NLI_LAUNCH_CHECK();
-    double sum[kSummary];
-    NLI_HIP(hipMemcpyAsync(sum, w.scalars.p, sizeof(sum), hipMemcpyDeviceToHost, s));
-    NLI_HIP(hipStreamSynchronize(s));
+    const volatile double *sum = wait_summary(w, sa, s);
     if (stats)
         for (int c = 0; c < nc; c++) {
             stats->iterations[c] = (int)sum[kSumDone + c];
@@ -1211,11 +1247,12 @@ double update(int kind, int patch, float sigma, float lambda, const CGParams &cg
         const int nb = (int)grid_for(n);
         w.scalars.ensure(kSummary);
         w.partials.ensure((size_t)kMaxNC * nb + 8);
-        double *result = w.scalars.p + kSumTotal;
         k_write_back<<<nb, kBlock, 0, s>>>(image, Cs, c0, nc, hole_pts, n, newv, w.pflag.p, w.partials.p);
-        k_sum_partials<<<1, kBlock, 0, s>>>(w.partials.p, nb, w.scalars.p, nullptr, 0, nullptr, 0);
+        const SumArgs sa = sum_args(w);
+        k_sum_partials<<<1, kBlock, 0, s>>>(w.partials.p, nb, w.scalars.p, nullptr, 0, nullptr, 0, sa.host_out,
+                                            sa.host_stamp, sa.stamp);
         NLI_LAUNCH_CHECK();
-        return read_scalar(result, s);
+        return wait_summary(w, sa, s)[kSumTotal];
     }
     throw std::invalid_argument("unknown image updating kind");
 }

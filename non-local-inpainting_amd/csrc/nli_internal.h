@@ -213,12 +213,16 @@ struct UpdWork {
     int *done_host = nullptr; // pinned: pipelined CG convergence polls (flags + stamp per slot)
     int *done_dev = nullptr;  // the same memory as the kernels address it
     unsigned poll_seq = 1;    // next unused poll stamp (stamps are never reused)
+    double *sum_host = nullptr;   // pinned: the update's summary, then its stamp
+    double *sum_dev = nullptr;
+    unsigned sum_seq = 0;         // stamp of the last summary asked for
     UpdWork() = default;
     UpdWork(const UpdWork &) = delete;
     UpdWork &operator=(const UpdWork &) = delete;
     ~UpdWork()
     {
         if (done_host) (void)hipHostFree(done_host);
+        if (sum_host) (void)hipHostFree(sum_host);
     }
     DBuf<float> weights, grad, conf_one;
     DBuf<double> a1, a2, f1, f2, F1, F2, partials, scalars;
