@@ -366,7 +366,7 @@ def main():
         # the colour path of the reference's command-line program (0-255 RGB <-> L*a*b*
         # around process(), main.cpp:117, 307): both conversions over the batch's images
         # (scaled to 0-255), device-resident, event-timed; not part of the timed steps
-        d_rgb = [ctx.to_device((w.image * 255.0).astype(np.float32)) for w in wls]
+        d_rgb = [ctx.to_device(np.rint(w.image * 255.0).astype(np.float32)) for w in wls]   # 8-bit samples
         d_lab = ctx.dev_alloc(W * H * C * 4)
         d_back = ctx.dev_alloc(W * H * C * 4)
         for d in d_rgb[:1]:
@@ -387,10 +387,10 @@ def main():
                              achieved_gb_s=gbs, frac_of_hbm_peak=gbs / HBM_PEAK_GBS)
         for p in d_rgb + [d_lab, d_back]:
             ctx.dev_free(p)
-        colour = dict(image=f"{W}x{H}x3 float32, 0-255", kernels=res,
+        colour = dict(image=f"{W}x{H}x3 float32 holding 8-bit sample values 0-255", kernels=res,
                       note="24 algorithmic bytes per pixel (3 floats in, 3 out); each of the reference's powf "
-                           "calls (3 per pixel per direction) is evaluated to the correctly rounded float in FP64, "
-                           "which bounds the kernels rather than HBM")
+                           "calls (3 per pixel per direction) is evaluated to the correctly rounded float in FP64 "
+                           "(Newton-refined roots), which bounds the kernels rather than HBM")
         log(f"colour path: {colour}")
 
     cpu = None

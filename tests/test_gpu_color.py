@@ -87,6 +87,20 @@ def test_pass_through_four_channels_and_odd_sizes(ctx, oracle):
     compare("rgb_to_lab 1 pixel", ctx.rgb_to_lab(one), oracle.rgb_to_lab(one), LAB_TOL)
 
 
+def test_table_edges(ctx, oracle):
+    """rgb_to_lab takes the gamma expansion of the integral values 0..255 from a table
+    computed by the same function; -0.0, values just off an integer, 255 and above, and
+    negative values are computed directly.  All of them against the oracle."""
+    v = np.array([-0.0, 0.0, 1.0, np.nextafter(np.float32(1.0), np.float32(2.0)), 10.0, 10.5, 11.0,
+                  np.nextafter(np.float32(255.0), np.float32(0.0)), 255.0, 256.0, 300.0, -1.0, -255.0],
+                 np.float32)
+    img = np.stack(np.meshgrid(v, v, v, indexing="ij"), axis=-1).reshape(len(v), -1, 3)
+    got, want = ctx.rgb_to_lab(img), oracle.rgb_to_lab(img)
+    compare("rgb_to_lab table edges", got, want, LAB_TOL)
+    sel = (img == 0).all(axis=2) & np.signbit(img).any(axis=2)
+    assert np.array_equal(np.signbit(got[sel]), np.signbit(want[sel]))   # -0.0 keeps its sign path
+
+
 def test_device_forms_equal_host_forms(ctx):
     img = colour_cases(7)
     H, W, _ = img.shape
