@@ -46,3 +46,27 @@ def test_plain_dropin_rejects_rgbd_distance_on_rgb(ctx, nli):
     img = np.full((40, 40, 3), 0.5, np.float32)
     with pytest.raises(nli.NliError):
         ctx.inpaint_plain(img, np.pad(np.ones((10, 10), np.uint8), 15), nli.Inpainting(method="nldepth", scales=1))
+
+
+def test_kernel_timing_all_and_sampled(ctx, syn, nli):
+    """nli_profile_enable(_sampled): with every sweep timed, every sweep of the run is
+    counted; timing one sweep in 4 counts every 4th of the same launches (the first, the
+    fifth, ...); the algorithmic work of the timed sweeps is positive."""
+    img = syn.texture(160, 128, 3, 4, 2)
+    hole = syn.ellipse_holes(160, 128, 3, 0.12, 2)
+    p = nli.Inpainting(method="nlpoisson", patch=9, iters=3, scales=2, coarse=0.5, cg_tol=1e-4, cg_limit=50, seed=2)
+    ctx.inpaint(img, hole, p)   # autotuned once, so both runs below launch the same sweeps
+    ctx.profile(["pm_sweep"])
+    ctx.inpaint(img, hole, p)
+    full = ctx.profile_read("pm_sweep")
+    ctx.profile(["pm_sweep"], period=4)
+    ctx.inpaint(img, hole, p)
+    part = ctx.profile_read("pm_sweep")
+    ctx.profile(False)
+    print(f"sweeps timed: all {full['launches']}, one in 4 {part['launches']}; "
+          f"average {full['ms'] / full['launches'] * 1e3:.1f} vs {part['ms'] / part['launches'] * 1e3:.1f} us")
+    assert full["launches"] > 8 and full["work"] > 0
+    assert part["launches"] == (full["launches"] + 3) // 4
+    assert part["work"] > 0
+    with pytest.raises(ValueError):
+        ctx.profile(["pm_sweep"], period=0)
