@@ -31,6 +31,18 @@ __global__ void k_fill_f32(float *p, float v, size_t n)
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+// one or two NNF fields set to (-1, -1) and their distances to FLT_MAX (n pixels each)
+__global__ void k_fill_fields(int2 *n0, float *d0, int2 *n1, float *d1, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    n0[i] = make_int2(-1, -1);
+    d0[i] = FLT_MAX;
+    if (n1) {
+        n1[i] = make_int2(-1, -1);
+        d1[i] = FLT_MAX;
+    }
+}
 __global__ void k_fill_f64(double *p, double v, size_t n)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -320,6 +332,12 @@ void fill_i32(int32_t *p, int32_t v, size_t n, hipStream_t s)
 {
     if (!n) return;
     k_fill_i32<<<grid_for(n), kBlock, 0, s>>>(p, v, n);
+    NLI_LAUNCH_CHECK();
+}
+void fill_fields(int32_t *n0, float *d0, int32_t *n1, float *d1, size_t n, hipStream_t s)
+{
+    if (!n) return;
+    k_fill_fields<<<grid_for(n), kBlock, 0, s>>>((int2 *)n0, d0, (int2 *)n1, d1, n);
     NLI_LAUNCH_CHECK();
 }
 void fill_f32(float *p, float v, size_t n, hipStream_t s)

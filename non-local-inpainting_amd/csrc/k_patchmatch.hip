@@ -1166,12 +1166,7 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     float *d_cur = (K % 2 == 0) ? dA : dB;
     int2 *n_oth = (int2 *)((K % 2 == 0) ? nB : out_nnf);
     float *d_oth = (K % 2 == 0) ? dB : dA;
-    fill_i32((int32_t *)n_cur, -1, 2 * tn, s);
-    fill_f32(d_cur, FLT_MAX, tn, s);
-    if (K > 0) {
-        fill_i32((int32_t *)n_oth, -1, 2 * tn, s);
-        fill_f32(d_oth, FLT_MAX, tn, s);
-    }
+    fill_fields((int32_t *)n_cur, d_cur, K > 0 ? (int32_t *)n_oth : nullptr, K > 0 ? d_oth : nullptr, tn, s);
     if (pr.ntiles == 0) {
         if (K % 2 == 1) {   // result must still be in out_nnf / dA
             NLI_HIP(hipMemcpyAsync(out_nnf, n_cur, 2 * tn * sizeof(int32_t), hipMemcpyDeviceToDevice, s));

@@ -125,6 +125,8 @@ void fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 void fill_i32(int32_t *p, int32_t v, size_t n, hipStream_t s);
 void fill_f32(float *p, float v, size_t n, hipStream_t s);
 void fill_f64(double *p, double v, size_t n, hipStream_t s);
+// NNF fields n0 (and n1, if not null) of n pixels to (-1, -1), distances d0 (d1) to FLT_MAX
+void fill_fields(int32_t *n0, float *d0, int32_t *n1, float *d1, size_t n, hipStream_t s);
 
 // ---- gradient (k_patchmatch.hip) ----
 void gradient(const float *img, int W, int H, int C, float *grad, hipStream_t s);
