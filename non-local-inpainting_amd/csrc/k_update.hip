@@ -604,11 +604,10 @@ struct Stencil {
     double at, al, ab, ar, sum, a1;
 };
 
-// list index of every hole pixel; block 0 also resets the CG's per-channel stop flags
-__global__ void k_lidx(const int *pts, int n, int *lidx, int *done_at)
+// list index of every hole pixel
+__global__ void k_lidx(const int *pts, int n, int *lidx)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x < kMaxNC) done_at[threadIdx.x] = -1;
     if (k < n) lidx[pts[k]] = k;
 }
 
@@ -634,9 +633,10 @@ __global__ void k_cg_prepare(const int *__restrict__ pts, int n, const int *__re
                              const double *__restrict__ f1, const double *__restrict__ F1,
                              const double *__restrict__ F2, int div_active, int W, int H,
                              Stencil *__restrict__ st, int4 *__restrict__ nbr, double *__restrict__ b,
-                             double *__restrict__ x0)
+                             double *__restrict__ x0, int *__restrict__ done_at)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < kMaxNC) done_at[threadIdx.x] = -1;   // the CG's stop flags
     if (k >= n) return;
     const int p = pts[k];
     const int x = p % W, y = p / W;
@@ -1116,10 +1116,19 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
             for (int c = 0; c < nc; c++) { stats->iterations[c] = 0; stats->nr0[c] = 0; stats->nr[c] = 0; }
         return 0.0;
     }
+    // the list-index map is rebuilt only when the hole point list changes (points_changed)
     w.lidx.ensure(np);
-    fill_i32(w.lidx.p, -1, np, s);
-    k_lidx<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p, w.flag.p + 2);
-    NLI_LAUNCH_CHECK();
+    if (w.lidx_version != w.pts_version || w.lidx_of != w.lidx.p || w.lidx_pts != hole_pts || w.lidx_n != n ||
+        w.lidx_np != np) {
+        fill_i32(w.lidx.p, -1, np, s);
+        k_lidx<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p);
+        NLI_LAUNCH_CHECK();
+        w.lidx_version = w.pts_version;
+        w.lidx_of = w.lidx.p;
+        w.lidx_pts = hole_pts;
+        w.lidx_n = n;
+        w.lidx_np = np;
+    }
     prof::begin(TK_POISSON_COEF, s);
     {
         const dim3 grid(grid_for(n), 3);
@@ -1145,7 +1154,7 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
     w.x0.ensure((size_t)n * nc);
     k_cg_prepare<<<grid_for(n), kBlock, 0, s>>>(hole_pts, n, w.lidx.p, image, orig, Cs, c0, nc, w.a1.p, w.a2.p,
                                                 w.f1.p, w.F1.p, w.F2.p, lambda != 1 ? 1 : 0, W, H, st, nbr, w.b.p,
-                                                w.x0.p);
+                                                w.x0.p, w.flag.p + 2);
     NLI_LAUNCH_CHECK();
     if (cg.precision == 0) {
         w.x1.ensure((size_t)n * nc);
@@ -1198,6 +1207,8 @@ double poisson(int patch, float sigma, float lambda, const CGParams &cg, float *
 }
 
 }  // namespace
+
+void points_changed(UpdWork &w) { w.pts_version++; }
 
 double update(int kind, int patch, float sigma, float lambda, const CGParams &cg, float *image, const float *orig,
               int Cs, int c0, int nc, const uint8_t *hole, const int *hole_pts, int n_hole, const uint8_t *ext,

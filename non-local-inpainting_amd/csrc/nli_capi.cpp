@@ -132,6 +132,7 @@ double do_update(nli_ctx *ctx, const nli_upd_params *u, float *image, const floa
     ctx->counts.ensure(n / 1024 + 2);
     ctx->total.ensure(2);
     const int nh = nli::compact_indices(domain, (int)n, ctx->pts.p, ctx->counts.p, ctx->total.p, ctx->stream);
+    nli::points_changed(ctx->upd);
     nli::CGStats st{};
     double total = 0.0;
     int channels = 0;

@@ -200,6 +200,7 @@ void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask,
         float *img = dw.img[L].p;
         const uint8_t *hole = dw.mask[L].p;
         const int nh = compact_indices(hole, (int)n, dw.hole_pts.p, dw.counts.p, dw.total.p, s);
+        points_changed(uw);
         rep.width[L] = w;
         rep.height[L] = h;
         rep.hole_pixels[L] = nh;

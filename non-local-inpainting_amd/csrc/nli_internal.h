@@ -218,6 +218,12 @@ struct UpdWork {
     double *sum_host = nullptr;   // pinned: the update's summary, then its stamp
     double *sum_dev = nullptr;
     unsigned sum_seq = 0;         // stamp of the last summary asked for
+    // the list-index map (lidx) is kept while the hole point list is unchanged: callers
+    // call points_changed() whenever they rewrite a point list
+    unsigned long long pts_version = 1, lidx_version = 0;
+    const int *lidx_of = nullptr, *lidx_pts = nullptr;
+    int lidx_n = -1;
+    size_t lidx_np = 0;
     UpdWork() = default;
     UpdWork(const UpdWork &) = delete;
     UpdWork &operator=(const UpdWork &) = delete;
@@ -251,6 +257,8 @@ struct CGStats {
 // stride Cs) is updated in place on channels [c0, c0+nc); orig is u0 (same stride).
 // hole_pts (n_hole, row-major) are the inpainting-domain points.  Returns the total
 // difference (synchronises the stream).
+// a caller's hole point list has been rewritten: the next update rebuilds its index map
+void points_changed(UpdWork &w);
 double update(int kind, int patch, float sigma, float lambda, const CGParams &cg, float *image,
               const float *orig, int Cs, int c0, int nc, const uint8_t *hole, const int *hole_pts,
               int n_hole, const uint8_t *ext, const int32_t *nnf, const float *conf, int W, int H,
