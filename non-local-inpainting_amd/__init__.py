@@ -601,6 +601,16 @@ class Context:
                                              ct.byref(st)))
         return out, d_out, stats_of(st)
 
+    def inpaint_rgb_dev(self, d_rgb: int, d_mask: int, d_out: int, W: int, H: int, params: "Inpainting",
+                        d_depth: int = 0, d_depth_out: int = 0):
+        """inpaint_rgb on device buffers (d_depth / d_depth_out 0: no depth plane)."""
+        p = params.to_struct()
+        st = InpaintStats()
+        V = ct.c_void_p
+        self._check(self.lib.nli_inpaint_rgb_dev(self.h, ct.byref(p), V(d_rgb), V(d_depth or None), V(d_mask), W, H,
+                                                 V(d_out), V(d_depth_out or None), ct.byref(st)))
+        return stats_of(st)
+
     def inpaint_rgb_plain(self, rgb, mask, params: Inpainting, depth=None):
         """main.cpp:115-309 through the C++ plugin classes (IOUtility, the plain drop-in);
         returns (rgb, depth or None, stats) like inpaint_rgb."""

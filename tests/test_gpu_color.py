@@ -164,6 +164,33 @@ def test_inpaint_rgb_with_depth(ctx, syn, nli):
     assert np.all(np.isfinite(dout))
 
 
+def test_inpaint_rgb_device_form(ctx, syn, nli):
+    """nli_inpaint_rgb_dev on device buffers equals nli_inpaint_rgb on host buffers, with
+    and without a depth plane."""
+    W, H = 112, 90
+    rgb = np.rint(syn.texture(W, H, 3, 3, 8) * 255.0).astype(np.float32)
+    depth = syn.depth_planes(W, H, 4, 8)
+    hole = syn.ellipse_holes(W, H, 3, 0.12, 8)
+    for method, d in (("nlpoisson", None), ("nldepth", depth)):
+        p = nli.Inpainting(method=method, patch=7, iters=3, scales=2, coarse=0.5, lambda_rgb=1.0, lambda_d=0.0,
+                           cg_tol=1e-4, cg_limit=60, seed=5)
+        out, dout, st = ctx.inpaint_rgb(rgb, hole, p, depth=d)
+        ptrs = [ctx.to_device(rgb), ctx.to_device(hole), ctx.dev_alloc(rgb.nbytes)]
+        if d is not None:
+            ptrs += [ctx.to_device(d), ctx.dev_alloc(d.nbytes)]
+        try:
+            st_dev = ctx.inpaint_rgb_dev(ptrs[0], ptrs[1], ptrs[2], W, H, p,
+                                         *((ptrs[3], ptrs[4]) if d is not None else ()))
+            got = ctx.from_device(ptrs[2], rgb.shape, np.float32)
+            gotd = ctx.from_device(ptrs[4], d.shape, np.float32) if d is not None else None
+        finally:
+            for q in ptrs:
+                ctx.dev_free(q)
+        assert bitexact(got, out) and st_dev.iterations == st.iterations
+        if d is not None:
+            assert bitexact(gotd, dout)
+
+
 def test_inpaint_rgb_parameter_rules_and_errors(ctx, oracle, syn, nli):
     rgb, hole = rgb_case(syn, 80, 64, 5)
     # coarse = 0: min(1.5 * patch / largest distance to the hole boundary, 1) (main.cpp:166-178)
