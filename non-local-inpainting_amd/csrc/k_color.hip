@@ -16,8 +16,10 @@
 //    with N = 3, 5, 12: an inverse N-th root by Newton's method from a hardware-log2/exp2
 //    seed, then the small factor x^(y - m/N) to third order.  The result is within
 //    about 1e-13 of x^y, so the float rounding is the correct one except within that
-//    distance of a rounding midpoint (measured against pow in double on 2.4e7 inputs: 1
-//    to 3 differ, each by one unit in the last place);
+//    distance of a rounding midpoint (a CPU emulation of the method against pow in
+//    double on 2.4e7 inputs: 1 to 3 differ, each by one unit in the last place; on the
+//    GPU the conversions agree with the reference as often as with exp2(y log2 x) in
+//    double, which pow_pos computes and which serves outside [2^-60, 2^60]);
 //  - the 8-bit values 0..255 of the gamma expansion through a table computed with the
 //    same function;
 //  - x^3 exactly: x*x is exact in double, x*x*x is rounded to odd (the exact error of
