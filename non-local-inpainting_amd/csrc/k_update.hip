@@ -757,16 +757,11 @@ __global__ void k_cg_init(CGArgs<T> a, const double *x0)
     block_reduce_store<NC>(acc, a.prr, a.nb);
 }
 
-// Per-channel scalars of an iteration, formed identically by every thread: the stop
-// test on tot = |r_it|^2 (phase A) and whether the channel still runs.  Block 0 records
-// |r_it|^2 and the stop; a block that reads done_at after block 0 set it in the same
-// kernel sees the channel stopped, which is what its own stop test says.
-template <int NC>
-struct CGScalars {
-    bool run[NC];
-    double coef[NC];   // beta (phase A) or alpha (phase B)
-    bool any;
-};
+// The per-channel scalars of an iteration (the stop test on |r_it|^2, beta, alpha, and
+// whether a channel still runs) are formed identically by every thread of every block.
+// Block 0 of k_cg_a records |r_it|^2 and the stop; a block that reads done_at after
+// block 0 set it in the same kernel sees the channel stopped, which is what its own
+// stop test says.
 
 // Loads of one unknown and its four stencil neighbours from two vectors (u, v);
 // issued before the grid totals are formed so that both memory round trips overlap.
