@@ -41,6 +41,12 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxNC = 4;
 constexpr int kPollSlot = kMaxNC + 1;   // a CG poll slot: kMaxNC stop flags, then its stamp
+
+// The update's kernels are short dependent chains (the CG above all) that share the CUs
+// with other streams' PatchMatch sweeps, which keep the vector units busy.  Their waves
+// run at raised issue priority so that they are not starved there (batch of 8 images,
+// mean of 5 runs: 4.49 -> 4.61 MPix/s).  A wave priority is local to the kernel's waves.
+__device__ inline void raise_wave_priority() { __builtin_amdgcn_s_setprio(3); }
 constexpr int kMaxP = 15;
 
 // scalar slots per channel
@@ -103,6 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restric
                                                          int hist_len, const int *__restrict__ done_at, int nc,
                                                          double *host_out, unsigned *host_stamp, unsigned stamp)
 {
+    raise_wave_priority();
     __shared__ double sh[kBlock];
     double s = 0.0;
     for (int b = threadIdx.x; b < nb; b += kBlock) s += partials[b];
@@ -430,6 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_poisson_coef(const int *__restrict__
                                double *__restrict__ a1, double *__restrict__ a2, double *__restrict__ f1,
                                double *__restrict__ F1, double *__restrict__ F2)
 {
+    raise_wave_priority();
     __shared__ float wl[kMaxP * kMaxP];
     for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
     __syncthreads();
@@ -465,6 +473,7 @@ __global__ __launch_bounds__(64) void k_poisson_coef16(const int *__restrict__ p
                                                        double *__restrict__ f1, double *__restrict__ F1,
                                                        double *__restrict__ F2)
 {
+    raise_wave_priority();
     static_assert(2 + 3 * NC <= kNLG, "one lane per accumulator");
     constexpr int kE = kNLG * SLOTS;
     __shared__ float wl[kMaxP * kMaxP];
@@ -607,6 +616,7 @@ struct Stencil {
 // list index of every hole pixel
 __global__ void k_lidx(const int *pts, int n, int *lidx)
 {
+    raise_wave_priority();
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) lidx[pts[k]] = k;
 }
@@ -635,6 +645,7 @@ __global__ void k_cg_prepare(const int *__restrict__ pts, int n, const int *__re
                              Stencil *__restrict__ st, int4 *__restrict__ nbr, double *__restrict__ b,
                              double *__restrict__ x0, int *__restrict__ done_at)
 {
+    raise_wave_priority();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x < kMaxNC) done_at[threadIdx.x] = -1;   // the CG's stop flags
     if (k >= n) return;
@@ -737,6 +748,7 @@ struct StencilOf {
 template <int NC, typename T>
 __global__ void k_cg_init(CGArgs<T> a, const double *x0)
 {
+    raise_wave_priority();
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[NC];
     for (int c = 0; c < NC; c++) acc[c] = 0.0;
@@ -789,6 +801,7 @@ struct Gather5 {
 template <int NC, typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_a(CGArgs<T> a, int it)
 {
+    raise_wave_priority();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = k < a.n;
     const T *dp = a.d[it & 1];
@@ -869,6 +882,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_a(CGArgs<T> a, int it)
 template <int NC, typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_b(CGArgs<T> a, int it)
 {
+    raise_wave_priority();
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = k < a.n;
     const T *xp = a.x[it & 1];
@@ -937,6 +951,7 @@ __global__ void k_poisson_write(float *__restrict__ img, const float *__restrict
                                 const int *__restrict__ pts, int n, T *x0, T *x1, const int *__restrict__ done_at,
                                 double *__restrict__ partials)
 {
+    raise_wave_priority();
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
     if (k < n) {
