@@ -21,6 +21,10 @@ import time
 
 import numpy as np
 
+# kernel arguments in device memory (the HIP runtime's default on this hardware, made
+# explicit): the CG's short dependent kernels pay for every argument fetch otherwise
+os.environ.setdefault("HIP_FORCE_DEV_KERNARG", "1")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
