@@ -15,7 +15,8 @@
  *   ImageInpainting               the multiscale driver of image_inpainting.cpp:68-557,
  *                                 with every numerical step a C-ABI call
  *   IOUtility                     rgb_to_lab / lab_to_rgb on the GPU (nli_rgb_to_lab,
- *                                 nli_lab_to_rgb), cat / separate / get_depth on the host
+ *                                 nli_lab_to_rgb), cat / separate / get_depth on the host,
+ *                                 PNG files (read/write_rgb_image, read/write_mono_image)
  *
  * Differences from the reference interface, all deliberate:
  *   - PatchMatch::calculate is virtual and PatchMatch is abstract: the GPU subclass is
@@ -40,6 +41,7 @@
 #define NLI_PLUGINS_HPP_
 
 #include "nli.h"
+#include "nli_io.h"
 
 #include <cmath>
 #include <cstdint>
@@ -248,8 +250,42 @@ public:
  * rgb_to_lab / lab_to_rgb run on the GPU (nli_rgb_to_lab / nli_lab_to_rgb) and take the
  * Context every GPU object takes; images they do not convert come back as deep copies,
  * as the reference returns its argument.  cat / separate / get_depth are the reference's
- * host-side channel copies with its messages.  Image file I/O (iio) is not provided. */
+ * host-side channel copies with its messages.  The image files (read_rgb_image,
+ * read_mono_image, write_rgb_image, write_mono_image) are PNG only, through nli_io.h in
+ * libnli_plugins.so; where the reference prints a message and exits, these throw
+ * std::runtime_error with the message. */
 struct IOUtility {
+    static void io_check(int rc, const char *prefix = "")
+    {
+        if (rc != NLI_OK) throw std::runtime_error(std::string(prefix) + nli_io_last_error());
+    }
+    static Image<float> read_rgb_image(const std::string &name)   // io_utility.cpp:143-168
+    {
+        int W = 0, H = 0;
+        io_check(nli_image_info(name.c_str(), &W, &H, nullptr, nullptr), "read_rgb_image: ");
+        Image<float> img((unsigned)W, (unsigned)H, 3u);
+        io_check(nli_read_rgb_image(name.c_str(), img.raw(), (long)W * H * 3));
+        return img;
+    }
+    static Image<float> read_mono_image(const std::string &name)   // io_utility.cpp:119-141
+    {
+        int W = 0, H = 0;
+        io_check(nli_image_info(name.c_str(), &W, &H, nullptr, nullptr), "read_mono_image: ");
+        Image<float> img((unsigned)W, (unsigned)H, 1u);
+        io_check(nli_read_mono_image(name.c_str(), img.raw(), (long)W * H));
+        return img;
+    }
+    static void write_rgb_image(const std::string &name, FixedImage<float> image)   // io_utility.cpp:171-190
+    {
+        io_check(nli_write_rgb_image(name.c_str(), image.raw(), (int)image.get_size_x(), (int)image.get_size_y(),
+                                     (int)image.get_number_of_channels()));
+    }
+    static void write_mono_image(const std::string &name, FixedImage<float> image)   // io_utility.cpp:192-220
+    {
+        io_check(nli_write_mono_image(name.c_str(), image.raw(), (int)image.get_size_x(), (int)image.get_size_y(),
+                                      (int)image.get_number_of_channels()));
+    }
+
     static Image<float> rgb_to_lab(Context &ctx, FixedImage<float> image)   // io_utility.cpp:268-289
     {
         if (image.get_number_of_channels() != 3) return Image<float>(image);

@@ -27,6 +27,7 @@ NLI_E_ARGUMENT = -1
 NLI_E_DEVICE = -2
 NLI_E_EMPTY_SOURCE = -3
 NLI_E_INTERNAL = -4
+NLI_E_IO = -5
 
 DIST = {"l2": 0, "l1": 1, "l2combined": 2, "l2rgbd": 3}
 UPD = {"nlmeans": 0, "nlmedians": 1, "nlpoisson": 2, "nldepth": 3}
@@ -49,7 +50,10 @@ EXPORTS = [
     "nli_lab_to_rgb_dev", "nli_inpaint_rgb", "nli_inpaint_rgb_dev", "nli_profile_enable_sampled",
 ]
 # symbols of libnli_plugins.so (include/nli_plugins.hpp)
-PLUGIN_EXPORTS = ["nli_plain_inpaint", "nli_plain_inpaint_rgb", "nli_plain_last_error"]
+PLUGIN_EXPORTS = ["nli_plain_inpaint", "nli_plain_inpaint_rgb", "nli_plain_last_error",
+                  # include/nli_io.h
+                  "nli_image_info", "nli_read_image", "nli_read_rgb_image", "nli_read_mono_image",
+                  "nli_write_image", "nli_write_rgb_image", "nli_write_mono_image", "nli_io_last_error"]
 KERNELS = {"pm_sweep": 0, "pm_init": 1, "cg_iteration": 2, "poisson_coef": 3, "nl_gather": 4, "color": 6}
 
 
@@ -146,6 +150,10 @@ def load_plugins() -> ct.CDLL:
         raise ImportError(f"{PLUGINS_PATH} is not built: run __graft_entry__.build()")
     lib = ct.CDLL(PLUGINS_PATH)
     lib.nli_plain_last_error.restype = ct.c_char_p
+    lib.nli_io_last_error.restype = ct.c_char_p
+    lib.nli_read_image.argtypes = [ct.c_char_p, _f32p, ct.c_long]
+    lib.nli_read_rgb_image.argtypes = [ct.c_char_p, _f32p, ct.c_long]
+    lib.nli_read_mono_image.argtypes = [ct.c_char_p, _f32p, ct.c_long]
     _plugins = lib
     return lib
 
@@ -611,6 +619,27 @@ class Context:
                                                  V(d_out), V(d_depth_out or None), ct.byref(st)))
         return stats_of(st)
 
+    def inpaint_png(self, input_name: str, mask_name: str, output_name: str, params: Inpainting,
+                    depth_name: str = None, depth_output_name: str = None):
+        """main.cpp:115-312 with files: IOUtility::read_rgb_image (input), read_mono_image
+        (mask, taken where positive; depth), inpaint_rgb, write_rgb_image (output).  The
+        depth plane is written as a grey PNG (write_mono_image); the reference passes the
+        one-channel plane to write_rgb_image, which reads channels 1 and 2 from the next
+        pixels and past the end of the plane.  Returns the run's statistics."""
+        rgb = read_rgb_image(input_name)
+        mask_image = read_mono_image(mask_name)
+        if rgb.shape[:2] != mask_image.shape:
+            raise ValueError("ERROR: Input image and mask should be of the same size")
+        mask = (mask_image > 0).astype(np.uint8)
+        if not mask.any():
+            raise ValueError("ERROR: Mask should not be empty (all black)")
+        depth = read_mono_image(depth_name) if depth_name else None
+        out, d_out, stats = self.inpaint_rgb(rgb, mask, params, depth=depth)
+        write_rgb_image(output_name, out)
+        if depth is not None and depth_output_name:
+            write_mono_image(depth_output_name, d_out)
+        return stats
+
     def inpaint_rgb_plain(self, rgb, mask, params: Inpainting, depth=None):
         """main.cpp:115-309 through the C++ plugin classes (IOUtility, the plain drop-in);
         returns (rgb, depth or None, stats) like inpaint_rgb."""
@@ -653,6 +682,75 @@ class Context:
                 raise EmptySourceError(rc, msg)
             raise NliError(rc, msg)
         return out, stats_of(st)
+
+
+# ---- image files (include/nli_io.h; io_utility.cpp:119-220 over 3rdparty/iio/iio.c) ----
+def _io_check(rc: int):
+    if rc == NLI_OK:
+        return
+    msg = load_plugins().nli_io_last_error().decode(errors="replace")
+    if rc == NLI_E_ARGUMENT:
+        raise ValueError(msg)
+    if rc == NLI_E_IO:
+        raise OSError(msg)
+    raise NliError(rc, msg)
+
+
+def image_info(path: str):
+    """(W, H, channels, bit_depth) of a PNG as the reader returns it."""
+    lib = load_plugins()
+    v = [ct.c_int() for _ in range(4)]
+    _io_check(lib.nli_image_info(os.fsencode(path), *[ct.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+def _read(fn, path: str, channels):
+    W, H, C, _ = image_info(path)
+    C = C if channels is None else channels
+    out = np.zeros((H, W, C), np.float32)
+    _io_check(fn(os.fsencode(path), _ptr(out, _f32p), out.size))
+    return out
+
+
+def read_image(path: str):
+    """iio_read_image_float_vec: (H, W, C) float32, palette and low-bit grey expanded,
+    tRNS as alpha, 8-bit samples 0..255 and 16-bit samples 0..65535."""
+    return _read(load_plugins().nli_read_image, path, None)
+
+
+def read_rgb_image(path: str):
+    """IOUtility::read_rgb_image (io_utility.cpp:143-168): (H, W, 3) float32."""
+    return _read(load_plugins().nli_read_rgb_image, path, 3)
+
+
+def read_mono_image(path: str):
+    """IOUtility::read_mono_image (io_utility.cpp:119-141): (H, W) float32; 8-bit colour
+    becomes (uint8)(.299 R + .587 G + .114 B)."""
+    return _read(load_plugins().nli_read_mono_image, path, 1)[:, :, 0]
+
+
+def _write(fn, path: str, img):
+    img = _c(img, np.float32)
+    if img.ndim == 2:
+        img = img[:, :, None]
+    H, W, C = img.shape
+    _io_check(fn(os.fsencode(path), _ptr(img, _f32p), W, H, C))
+
+
+def write_image(path: str, img):
+    """iio_save_image_float_vec with a .png name: 8-bit samples (uint8)clamp(v + 0.5)."""
+    _write(load_plugins().nli_write_image, path, img)
+
+
+def write_rgb_image(path: str, img):
+    """IOUtility::write_rgb_image (io_utility.cpp:171-190): channels 0..2 as an RGB PNG."""
+    _write(load_plugins().nli_write_rgb_image, path, img)
+
+
+def write_mono_image(path: str, img):
+    """IOUtility::write_mono_image (io_utility.cpp:192-220): one channel, or the luminance
+    0.2126 R + 0.7152 G + 0.0722 B of three."""
+    _write(load_plugins().nli_write_mono_image, path, img)
 
 
 def stats_of(st: InpaintStats) -> RunStats:

@@ -164,6 +164,46 @@ def test_inpaint_rgb_with_depth(ctx, syn, nli):
     assert np.all(np.isfinite(dout))
 
 
+def test_inpaint_png_files_end_to_end(ctx, syn, nli, tmp_path):
+    """main.cpp:115-312 with files: PNG input, mask and depth written by a second encoder
+    (PIL), Context.inpaint_png, the output PNGs read back by PIL.  They hold exactly
+    inpaint_rgb's results taken to bytes as iio takes them (uint8)clamp(v + 0.5)."""
+    Image = pytest.importorskip("PIL.Image")
+    W, H = 120, 96
+    rgb8 = np.rint(syn.texture(W, H, 3, 3, 11) * 255.0).astype(np.uint8)
+    depth8 = np.rint(syn.depth_planes(W, H, 4, 11) * 255.0 / max(1e-6, syn.depth_planes(W, H, 4, 11).max()))
+    depth8 = depth8.astype(np.uint8)
+    hole = syn.ellipse_holes(W, H, 3, 0.12, 11)
+    Image.fromarray(rgb8, "RGB").save(tmp_path / "in.png")
+    Image.fromarray((hole > 0).astype(np.uint8) * 255, "L").convert("1").save(tmp_path / "mask.png")
+    Image.fromarray(depth8, "L").save(tmp_path / "depth.png")
+
+    def to_bytes(v):
+        t = 0.5 + np.asarray(v, np.float64)
+        return np.trunc(np.where(t > 0, np.where(t < 255, t, 255), 0)).astype(np.uint8)
+
+    for method, dname in (("nlpoisson", None), ("nldepth", "depth.png")):
+        p = nli.Inpainting(method=method, patch=7, iters=3, scales=2, coarse=0.5, lambda_rgb=1.0, lambda_d=0.0,
+                           seed=4)
+        out_name, dout_name = str(tmp_path / f"out_{method}.png"), str(tmp_path / f"dout_{method}.png")
+        st = ctx.inpaint_png(str(tmp_path / "in.png"), str(tmp_path / "mask.png"), out_name, p,
+                             depth_name=str(tmp_path / dname) if dname else None, depth_output_name=dout_name)
+        assert st.scales == 2
+        d = depth8.astype(np.float32) if dname else None
+        ref, dref, _ = ctx.inpaint_rgb(rgb8.astype(np.float32), (hole > 0).astype(np.uint8), p, depth=d)
+        got = np.asarray(Image.open(out_name))
+        assert bitexact(got, to_bytes(ref))
+        if dname:
+            assert bitexact(np.asarray(Image.open(dout_name)), to_bytes(dref))
+        print(f"{method}: {W}x{H} PNG in, PNG out; {int((hole > 0).sum())} hole pixels")
+    with pytest.raises(ValueError, match="same size"):
+        Image.fromarray(rgb8[:-1], "RGB").save(tmp_path / "small.png")
+        ctx.inpaint_png(str(tmp_path / "small.png"), str(tmp_path / "mask.png"), str(tmp_path / "x.png"), p)
+    Image.fromarray(np.zeros((H, W), np.uint8), "L").save(tmp_path / "black.png")
+    with pytest.raises(ValueError, match="should not be empty"):
+        ctx.inpaint_png(str(tmp_path / "in.png"), str(tmp_path / "black.png"), str(tmp_path / "x.png"), p)
+
+
 def test_inpaint_rgb_device_form(ctx, syn, nli):
     """nli_inpaint_rgb_dev on device buffers equals nli_inpaint_rgb on host buffers, with
     and without a depth plane."""
