@@ -7,7 +7,9 @@
 
 #include <zlib.h>
 
+#include <algorithm>
 #include <cmath>
+#include <new>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -210,7 +212,7 @@ void emit(const PngFile &p, const uint8_t *row, uint32_t i, float *px)
     }
 }
 
-int decode(const char *path, const PngFile &p, float *out)
+int decode(const char *path, const PngFile &p, std::vector<float> &img)
 {
     const int sc = p.src_channels(), oc = p.out_channels();
     const int bits = sc * p.depth, bpp = bits >= 8 ? bits / 8 : 1;
@@ -223,19 +225,38 @@ int decode(const char *path, const PngFile &p, float *out)
         uint32_t ph = p.h > (uint32_t)ps.y0 ? (p.h - ps.y0 + ps.dy - 1) / ps.dy : 0;
         if (pw && ph) total += (size_t)ph * (1 + row_bytes(pw, bits));
     }
+    // Deflate expands at most about 1032:1: a header that asks for more data than the
+    // file can hold is refused before anything is allocated for it.
+    if (total / 1032 > p.idat.size() + 64) return fail(NLI_E_IO, std::string(path) + ": not enough image data");
     std::vector<uint8_t> raw(total);
     z_stream z;
     std::memset(&z, 0, sizeof z);
     if (inflateInit(&z) != Z_OK) return fail(NLI_E_IO, "zlib: inflateInit failed");
+    // zlib counts in 32-bit units: feed input and output in slices of at most 1 GiB.
+    const size_t kSlice = (size_t)1 << 30;
+    size_t in_left = p.idat.size(), out_left = raw.size();
     z.next_in = const_cast<uint8_t *>(p.idat.data());
-    z.avail_in = (uInt)p.idat.size();
     z.next_out = raw.data();
-    z.avail_out = (uInt)raw.size();
     int zr = Z_OK;
-    while (z.avail_out > 0 && zr == Z_OK) zr = inflate(&z, Z_NO_FLUSH);
+    while ((out_left > 0 || z.avail_out > 0) && zr == Z_OK) {
+        if (z.avail_in == 0 && in_left > 0) {
+            z.avail_in = (uInt)std::min(in_left, kSlice);
+            in_left -= z.avail_in;
+        }
+        if (z.avail_out == 0) {
+            z.avail_out = (uInt)std::min(out_left, kSlice);
+            out_left -= z.avail_out;
+        }
+        zr = inflate(&z, Z_NO_FLUSH);
+        if (zr == Z_BUF_ERROR && z.avail_in == 0 && in_left == 0) break;   // input exhausted
+        if (zr == Z_BUF_ERROR) zr = Z_OK;
+    }
+    const bool short_data = out_left > 0 || z.avail_out > 0;
     inflateEnd(&z);
     // Compressed data beyond the image is ignored (libpng warns and goes on).
-    if (z.avail_out != 0) return fail(NLI_E_IO, std::string(path) + ": not enough image data");
+    if (short_data) return fail(NLI_E_IO, std::string(path) + ": not enough image data");
+    img.assign((size_t)p.w * p.h * oc, 0.0f);
+    float *out = img.data();
 
     size_t at = 0;
     for (int k = 0; k < npass; ++k) {
@@ -268,8 +289,10 @@ int load(const char *path, std::vector<float> &img, PngFile &p)
     if (!path) return fail(NLI_E_ARGUMENT, "path is NULL");
     int rc = parse(path, p);
     if (rc != NLI_OK) return rc;
-    img.assign((size_t)p.w * p.h * p.out_channels(), 0.0f);
-    return decode(path, p, img.data());
+    // Sized after decode's check of the data against the header (a lazy resize keeps a
+    // refused file from allocating the output).
+    img.clear();
+    return decode(path, p, img);
 }
 
 bool png_name(const char *path)
@@ -331,7 +354,9 @@ int encode(const char *path, const std::vector<uint8_t> &px, int W, int H, int C
     put32(ihdr, (uint32_t)W); put32(ihdr, (uint32_t)H);
     ihdr.insert(ihdr.end(), {8, (uint8_t)ctype, 0, 0, 0});
     chunk("IHDR", ihdr.data(), ihdr.size());
-    chunk("IDAT", zbuf.data(), zbuf.size());
+    // IDAT chunks of at most 1 MiB (a chunk length must stay below 2^31).
+    for (size_t at = 0; at < zbuf.size(); at += (size_t)1 << 20)
+        chunk("IDAT", zbuf.data() + at, std::min(zbuf.size() - at, (size_t)1 << 20));
     chunk("IEND", nullptr, 0);
 
     FILE *f = std::fopen(path, "wb");
@@ -359,105 +384,134 @@ int write_bytes(const char *path, const float *img, int W, int H, int C, int CO,
     return encode(path, px, W, H, CO);
 }
 
+// Runs an entry point's body; no exception crosses the C-ABI (a header that declares
+// a huge image can make an allocation fail).
+template <class F> int guarded(F &&body)
+{
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        return fail(NLI_E_IO, "out of memory");
+    } catch (const std::exception &e) {
+        return fail(NLI_E_INTERNAL, e.what());
+    } catch (...) {
+        return fail(NLI_E_INTERNAL, "unknown exception");
+    }
+}
+
 }  // namespace
 
 extern "C" {
 
 int nli_image_info(const char *path, int *W, int *H, int *channels, int *bit_depth)
 {
-    if (!path) return fail(NLI_E_ARGUMENT, "path is NULL");
-    PngFile p;
-    int rc = parse(path, p);
-    if (rc != NLI_OK) return rc;
-    if (W) *W = (int)p.w;
-    if (H) *H = (int)p.h;
-    if (channels) *channels = p.out_channels();
-    if (bit_depth) *bit_depth = p.out_depth();
-    return NLI_OK;
+    return guarded([&]() -> int {
+        if (!path) return fail(NLI_E_ARGUMENT, "path is NULL");
+        PngFile p;
+        int rc = parse(path, p);
+        if (rc != NLI_OK) return rc;
+        if (W) *W = (int)p.w;
+        if (H) *H = (int)p.h;
+        if (channels) *channels = p.out_channels();
+        if (bit_depth) *bit_depth = p.out_depth();
+        return NLI_OK;
+    });
 }
 
 int nli_read_image(const char *path, float *out, long capacity)
 {
-    if (!out) return fail(NLI_E_ARGUMENT, "out is NULL");
-    std::vector<float> img;
-    PngFile p;
-    int rc = load(path, img, p);
-    if (rc != NLI_OK) return rc;
-    if (capacity < (long)img.size()) return fail(NLI_E_ARGUMENT, "output buffer too small");
-    std::memcpy(out, img.data(), img.size() * sizeof(float));
-    return NLI_OK;
+    return guarded([&]() -> int {
+        if (!out) return fail(NLI_E_ARGUMENT, "out is NULL");
+        std::vector<float> img;
+        PngFile p;
+        int rc = load(path, img, p);
+        if (rc != NLI_OK) return rc;
+        if (capacity < (long)img.size()) return fail(NLI_E_ARGUMENT, "output buffer too small");
+        std::memcpy(out, img.data(), img.size() * sizeof(float));
+        return NLI_OK;
+    });
 }
 
 int nli_read_rgb_image(const char *path, float *rgb, long capacity)
 {
-    if (!rgb) return fail(NLI_E_ARGUMENT, "out is NULL");
-    std::vector<float> img;
-    PngFile p;
-    int rc = load(path, img, p);
-    if (rc != NLI_OK) return fail(rc, std::string("read_rgb_image: cannot read ") + path + " (" + t_err + ")");
-    const size_t n = (size_t)p.w * p.h;
-    if (capacity < (long)(n * 3)) return fail(NLI_E_ARGUMENT, "output buffer too small");
-    const int ch = p.out_channels();
-    for (size_t i = 0; i < n; ++i) {
-        const float *s = &img[i * ch];
-        rgb[3 * i + 0] = s[0];
-        rgb[3 * i + 1] = ch > 1 ? s[1] : s[0];
-        rgb[3 * i + 2] = ch > 2 ? s[2] : s[0];
-    }
-    return NLI_OK;
+    return guarded([&]() -> int {
+        if (!rgb) return fail(NLI_E_ARGUMENT, "out is NULL");
+        std::vector<float> img;
+        PngFile p;
+        int rc = load(path, img, p);
+        if (rc != NLI_OK) return fail(rc, std::string("read_rgb_image: cannot read ") + path + " (" + t_err + ")");
+        const size_t n = (size_t)p.w * p.h;
+        if (capacity < (long)(n * 3)) return fail(NLI_E_ARGUMENT, "output buffer too small");
+        const int ch = p.out_channels();
+        for (size_t i = 0; i < n; ++i) {
+            const float *s = &img[i * ch];
+            rgb[3 * i + 0] = s[0];
+            rgb[3 * i + 1] = ch > 1 ? s[1] : s[0];
+            rgb[3 * i + 2] = ch > 2 ? s[2] : s[0];
+        }
+        return NLI_OK;
+    });
 }
 
 int nli_read_mono_image(const char *path, float *out, long capacity)
 {
-    if (!out) return fail(NLI_E_ARGUMENT, "out is NULL");
-    std::vector<float> img;
-    PngFile p;
-    int rc = load(path, img, p);
-    if (rc != NLI_OK) return fail(rc, std::string("read_mono_image: cannot read ") + path + " (" + t_err + ")");
-    const size_t n = (size_t)p.w * p.h;
-    if (capacity < (long)n) return fail(NLI_E_ARGUMENT, "output buffer too small");
-    const int ch = p.out_channels();
-    if (ch == 2)
-        return fail(NLI_E_IO, std::string("read_mono_image: cannot read ") + path + " (non-scalarizable image)");
-    if (ch >= 3 && p.out_depth() == 16)
-        return fail(NLI_E_IO, std::string("read_mono_image: cannot read ") + path + " (uncolorize type not supported)");
-    for (size_t i = 0; i < n; ++i) {
-        const float *s = &img[i * ch];
-        if (ch == 1) {
-            out[i] = s[0];
-        } else {
-            // iio.c:883 / 916: a uint8_t from a double expression (truncation).
-            double l = .299 * (double)s[0] + .587 * (double)s[1] + .114 * (double)s[2];
-            out[i] = (float)(uint8_t)l;
+    return guarded([&]() -> int {
+        if (!out) return fail(NLI_E_ARGUMENT, "out is NULL");
+        std::vector<float> img;
+        PngFile p;
+        int rc = load(path, img, p);
+        if (rc != NLI_OK) return fail(rc, std::string("read_mono_image: cannot read ") + path + " (" + t_err + ")");
+        const size_t n = (size_t)p.w * p.h;
+        if (capacity < (long)n) return fail(NLI_E_ARGUMENT, "output buffer too small");
+        const int ch = p.out_channels();
+        if (ch == 2)
+            return fail(NLI_E_IO, std::string("read_mono_image: cannot read ") + path + " (non-scalarizable image)");
+        if (ch >= 3 && p.out_depth() == 16)
+            return fail(NLI_E_IO, std::string("read_mono_image: cannot read ") + path + " (uncolorize type not supported)");
+        for (size_t i = 0; i < n; ++i) {
+            const float *s = &img[i * ch];
+            if (ch == 1) {
+                out[i] = s[0];
+            } else {
+                // iio.c:883 / 916: a uint8_t from a double expression (truncation).
+                double l = .299 * (double)s[0] + .587 * (double)s[1] + .114 * (double)s[2];
+                out[i] = (float)(uint8_t)l;
+            }
         }
-    }
-    return NLI_OK;
+        return NLI_OK;
+    });
 }
 
 int nli_write_image(const char *path, const float *img, int W, int H, int C)
 {
-    int rc = check_write_args(path, img, W, H, C);
-    if (rc != NLI_OK) return rc;
-    return write_bytes(path, img, W, H, C, C, [](const float *px, int c) { return px[c]; });
+    return guarded([&]() -> int {
+        int rc = check_write_args(path, img, W, H, C);
+        if (rc != NLI_OK) return rc;
+        return write_bytes(path, img, W, H, C, C, [](const float *px, int c) { return px[c]; });
+    });
 }
 
 int nli_write_rgb_image(const char *path, const float *img, int W, int H, int C)
 {
-    int rc = check_write_args(path, img, W, H, C);
-    if (rc != NLI_OK) return rc;
-    if (C < 3) return fail(NLI_E_ARGUMENT, "write_rgb_image: the image needs at least 3 channels");
-    return write_bytes(path, img, W, H, C, 3, [](const float *px, int c) { return px[c]; });
+    return guarded([&]() -> int {
+        int rc = check_write_args(path, img, W, H, C);
+        if (rc != NLI_OK) return rc;
+        if (C < 3) return fail(NLI_E_ARGUMENT, "write_rgb_image: the image needs at least 3 channels");
+        return write_bytes(path, img, W, H, C, 3, [](const float *px, int c) { return px[c]; });
+    });
 }
 
 int nli_write_mono_image(const char *path, const float *img, int W, int H, int C)
 {
-    int rc = check_write_args(path, img, W, H, C);
-    if (rc != NLI_OK) return rc;
-    if (C != 1 && C != 3) return fail(NLI_E_ARGUMENT, "write_mono_image: the image needs 1 or 3 channels");
-    if (C == 1) return write_bytes(path, img, W, H, 1, 1, [](const float *px, int) { return px[0]; });
-    return write_bytes(path, img, W, H, 3, 1, [](const float *px, int) {
-        // io_utility.cpp:212-216: a double expression stored into a float sample.
-        return (float)(0.2126 * (double)px[0] + 0.7152 * (double)px[1] + 0.0722 * (double)px[2]);
+    return guarded([&]() -> int {
+        int rc = check_write_args(path, img, W, H, C);
+        if (rc != NLI_OK) return rc;
+        if (C != 1 && C != 3) return fail(NLI_E_ARGUMENT, "write_mono_image: the image needs 1 or 3 channels");
+        if (C == 1) return write_bytes(path, img, W, H, 1, 1, [](const float *px, int) { return px[0]; });
+        return write_bytes(path, img, W, H, 3, 1, [](const float *px, int) {
+            // io_utility.cpp:212-216: a double expression stored into a float sample.
+            return (float)(0.2126 * (double)px[0] + 0.7152 * (double)px[1] + 0.0722 * (double)px[2]);
+        });
     });
 }
 

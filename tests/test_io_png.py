@@ -321,6 +321,15 @@ def test_damaged_files(nli, tmp_path):
     q = str(tmp_path / "anc.png")
     write_png(q, rng.integers(0, 256, (5, 6, 3)), 2, 8, extra=anc)
     assert nli.read_image(q).shape == (5, 6, 3)
+    # a header far larger than its data is refused before the image is allocated
+    raw = zlib.compress(b"\0" * 100)
+    huge = (b"\x89PNG\r\n\x1a\n" + _chunk(b"IHDR", struct.pack(">IIBBBBB", 46000, 46000, 16, 6, 0, 0, 0))
+            + _chunk(b"IDAT", raw) + _chunk(b"IEND", b""))
+    q = variant("huge.png", huge)
+    assert nli.image_info(q) == (46000, 46000, 4, 16)
+    one = np.zeros(1, np.float32)
+    with pytest.raises(OSError, match="not enough image data"):   # checked before the capacity
+        nli._io_check(nli.load_plugins().nli_read_image(q.encode(), one.ctypes.data_as(nli._f32p), 1))
     # a buffer too small is refused before anything is written
     lib = nli.load_plugins()
     out = np.zeros(10, np.float32)
