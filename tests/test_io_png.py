@@ -389,3 +389,48 @@ int main(int argc, char **argv) {
     np.testing.assert_array_equal(pil_array(tmp_path / "b.png"), a)
     luma = np.trunc(.299 * a[..., 0].astype(np.float64) + .587 * a[..., 1] + .114 * a[..., 2])
     np.testing.assert_array_equal(pil_array(tmp_path / "m.png")[:, :, 0], luma.astype(np.uint8))
+
+
+def test_mutated_files_are_decoded_or_refused(nli, tmp_path):
+    """Damaged files (bit flips, truncation, rewritten header fields; CRCs repaired on most
+    so the damage reaches the decoder) come back decoded or refused with a code, never a
+    crash."""
+    rng = np.random.default_rng(21)
+    seeds = []
+    for k, (ctype, depth, il) in enumerate([(0, 2, True), (2, 16, False), (3, 4, True), (6, 8, False)]):
+        p = str(tmp_path / f"seed{k}.png")
+        plte = rng.integers(0, 256, (16, 3)) if ctype == 3 else None
+        write_png(p, rng.integers(0, (1 << depth), (9, 10, CTYPE_CH[ctype])), ctype, depth, il, rng, plte=plte)
+        seeds.append(open(p, "rb").read())
+
+    def fix_crcs(m):
+        at = 8
+        while at + 12 <= len(m):
+            n = struct.unpack(">I", bytes(m[at:at + 4]))[0]
+            if n > len(m) - at - 12:
+                break
+            m[at + 8 + n:at + 12 + n] = struct.pack(">I", zlib.crc32(bytes(m[at + 4:at + 8 + n])) & 0xffffffff)
+            at += 12 + n
+
+    lib = nli.load_plugins()
+    buf = np.zeros(4 * 64 * 64, np.float32)
+    q = str(tmp_path / "m.png")
+    counts = {}
+    for it in range(1500):
+        m = bytearray(seeds[it % len(seeds)])
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            for _ in range(int(rng.integers(1, 6))):
+                m[int(rng.integers(8, len(m)))] ^= 1 << int(rng.integers(0, 8))
+        elif kind == 1:
+            m = m[:int(rng.integers(8, len(m)))]
+        else:
+            m[int(rng.integers(16, 29))] = int(rng.integers(0, 256))
+        if rng.random() < 0.75:
+            fix_crcs(m)
+        with open(q, "wb") as f:
+            f.write(m)
+        rc = lib.nli_read_image(q.encode(), buf.ctypes.data_as(nli._f32p), buf.size)
+        assert rc in (nli.NLI_OK, nli.NLI_E_IO, nli.NLI_E_ARGUMENT)
+        counts[rc] = counts.get(rc, 0) + 1
+    assert counts.get(nli.NLI_OK, 0) > 50 and counts.get(nli.NLI_E_IO, 0) > 50
