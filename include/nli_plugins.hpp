@@ -16,7 +16,8 @@
  *                                 with every numerical step a C-ABI call
  *   IOUtility                     rgb_to_lab / lab_to_rgb on the GPU (nli_rgb_to_lab,
  *                                 nli_lab_to_rgb), cat / separate / get_depth on the host,
- *                                 PNG files (read/write_rgb_image, read/write_mono_image)
+ *                                 PNG files (read/write_rgb_image, read/write_mono_image),
+ *                                 mask_to_greyscale, probability_to_greyscale
  *
  * Differences from the reference interface, all deliberate:
  *   - PatchMatch::calculate is virtual and PatchMatch is abstract: the GPU subclass is
@@ -286,6 +287,22 @@ struct IOUtility {
                                       (int)image.get_number_of_channels()));
     }
 
+    // Masked points white (255), the rest black (io_utility.cpp:223-236).
+    static Image<float> mask_to_greyscale(FixedMask mask)
+    {
+        Image<float> out(mask.get_size(), 1u, 0.0f);
+        for (const Point &p : mask.get_masked_points()) out((unsigned)p.x, (unsigned)p.y) = 255.0f;
+        return out;
+    }
+    // [0, 1] to [0, 255], clamped in double (io_utility.cpp:243-256).
+    static Image<float> probability_to_greyscale(FixedImage<float> probabilities)
+    {
+        Image<float> out(probabilities.get_size(), 1u, 0.0f);
+        for (unsigned y = 0; y < probabilities.get_size_y(); y++)
+            for (unsigned x = 0; x < probabilities.get_size_x(); x++)
+                out(x, y) = (float)(255.0 * std::fmax(0.0, std::fmin(1.0, (double)probabilities(x, y))));
+        return out;
+    }
     static Image<float> rgb_to_lab(Context &ctx, FixedImage<float> image)   // io_utility.cpp:268-289
     {
         if (image.get_number_of_channels() != 3) return Image<float>(image);

@@ -371,6 +371,12 @@ int main(int argc, char **argv) {
     nli::IOUtility::write_rgb_image(argv[2], rgb);
     nli::Image<float> mono = nli::IOUtility::read_mono_image(argv[1]);
     nli::IOUtility::write_mono_image(argv[3], mono);
+    nli::Mask m(5, 4);
+    m.mask(1, 0); m.mask(4, 3);
+    nli::IOUtility::write_mono_image(argv[5], nli::IOUtility::mask_to_greyscale(m));
+    nli::Image<float> pr(4, 1);
+    pr(0, 0) = -1.0f; pr(1, 0) = 0.25f; pr(2, 0) = 2.0f; pr(3, 0) = 0.5f;
+    nli::IOUtility::write_mono_image(argv[6], nli::IOUtility::probability_to_greyscale(pr));
     try { nli::IOUtility::read_rgb_image(argv[4]); return 3; }
     catch (const std::runtime_error &e) { std::printf("%s\n", e.what()); }
     return 0;
@@ -383,12 +389,18 @@ int main(int argc, char **argv) {
     a = rng.integers(0, 256, (21, 19, 3)).astype(np.uint8)
     PIL.fromarray(a, "RGB").save(tmp_path / "a.png")
     r = subprocess.run([exe, str(tmp_path / "a.png"), str(tmp_path / "b.png"), str(tmp_path / "m.png"),
-                        str(tmp_path / "none.png")], capture_output=True, text=True, timeout=60)
+                        str(tmp_path / "none.png"), str(tmp_path / "mask.png"), str(tmp_path / "prob.png")],
+                       capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     assert "read_rgb_image: cannot read" in r.stdout
     np.testing.assert_array_equal(pil_array(tmp_path / "b.png"), a)
     luma = np.trunc(.299 * a[..., 0].astype(np.float64) + .587 * a[..., 1] + .114 * a[..., 2])
     np.testing.assert_array_equal(pil_array(tmp_path / "m.png")[:, :, 0], luma.astype(np.uint8))
+    want = np.zeros((4, 5), np.uint8)
+    want[0, 1] = want[3, 4] = 255
+    np.testing.assert_array_equal(pil_array(tmp_path / "mask.png")[:, :, 0], want)
+    # 255 * clamp(p, 0, 1): 0, 63.75 -> 64 (rounded on writing), 255, 127.5 -> 128
+    np.testing.assert_array_equal(pil_array(tmp_path / "prob.png")[:, :, 0], [[0, 64, 255, 128]])
 
 
 def test_mutated_files_are_decoded_or_refused(nli, tmp_path):
