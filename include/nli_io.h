@@ -66,6 +66,23 @@ int nli_write_rgb_image(const char *path, const float *img, int W, int H, int C)
  * counts are refused: the reference leaves those samples unset. */
 int nli_write_mono_image(const char *path, const float *img, int W, int H, int C);
 
+/* IOUtility::read_pgm_image (io_utility.cpp:26-74; stdio, no iio): "P2" (ASCII) or "P5"
+ * (one byte per sample, whatever the maximum value says).  Whitespace and '#' comments
+ * are skipped between the header fields and also before the samples, so binary samples
+ * at the start that are whitespace bytes are skipped, as there; a binary file that ends
+ * early gives -1 for each missing sample (fgetc's EOF).  nli_pgm_info gives the size;
+ * capacity >= W*H.  The reference returns an empty image for a file it cannot open or
+ * whose magic is not P2/P5 and exits on a bad ASCII sample: here both are NLI_E_IO. */
+int nli_pgm_info(const char *path, int *W, int *H);
+int nli_read_pgm_image(const char *path, float *out, long capacity);
+
+/* IOUtility::write_pgm_image (io_utility.cpp:80-117): "P5", maximum 255, one byte per
+ * pixel: (char)v for C = 1, (char)(0.2126 R + 0.7152 G + 0.0722 B) in double for C = 3,
+ * the header alone for other C.  (char) of a value outside char is undefined in C++;
+ * the bytes are those x86-64 builds of the reference write: truncation to a 32-bit
+ * integer (0x80000000 for NaN and out-of-range values), its low byte. */
+int nli_write_pgm_image(const char *path, const float *img, int W, int H, int C);
+
 /* Message of the calling thread's last failed nli_io call ("" if none). */
 const char *nli_io_last_error(void);
 

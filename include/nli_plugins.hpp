@@ -17,7 +17,8 @@
  *   IOUtility                     rgb_to_lab / lab_to_rgb on the GPU (nli_rgb_to_lab,
  *                                 nli_lab_to_rgb), cat / separate / get_depth on the host,
  *                                 PNG files (read/write_rgb_image, read/write_mono_image),
- *                                 mask_to_greyscale, probability_to_greyscale
+ *                                 PGM files (read/write_pgm_image), mask_to_greyscale,
+ *                                 probability_to_greyscale
  *
  * Differences from the reference interface, all deliberate:
  *   - PatchMatch::calculate is virtual and PatchMatch is abstract: the GPU subclass is
@@ -287,6 +288,22 @@ struct IOUtility {
                                       (int)image.get_number_of_channels()));
     }
 
+    // PGM without iio (io_utility.cpp:26-117): an empty image where the reference
+    // returns one (a file it cannot read); a bad ASCII sample, where the reference exits,
+    // throws.
+    static Image<float> read_pgm_image(const std::string &name)
+    {
+        int W = 0, H = 0;
+        if (nli_pgm_info(name.c_str(), &W, &H) != NLI_OK) return Image<float>();
+        Image<float> img((unsigned)W, (unsigned)H, 1u);
+        io_check(nli_read_pgm_image(name.c_str(), img.raw(), (long)W * H), "read_pgm_image: ");
+        return img;
+    }
+    static void write_pgm_image(const std::string &name, FixedImage<float> image)
+    {
+        io_check(nli_write_pgm_image(name.c_str(), image.raw(), (int)image.get_size_x(), (int)image.get_size_y(),
+                                     (int)image.get_number_of_channels()));
+    }
     // Masked points white (255), the rest black (io_utility.cpp:223-236).
     static Image<float> mask_to_greyscale(FixedMask mask)
     {

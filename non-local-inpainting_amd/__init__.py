@@ -53,7 +53,8 @@ EXPORTS = [
 PLUGIN_EXPORTS = ["nli_plain_inpaint", "nli_plain_inpaint_rgb", "nli_plain_last_error",
                   # include/nli_io.h
                   "nli_image_info", "nli_read_image", "nli_read_rgb_image", "nli_read_mono_image",
-                  "nli_write_image", "nli_write_rgb_image", "nli_write_mono_image", "nli_io_last_error"]
+                  "nli_write_image", "nli_write_rgb_image", "nli_write_mono_image", "nli_io_last_error",
+                  "nli_pgm_info", "nli_read_pgm_image", "nli_write_pgm_image"]
 KERNELS = {"pm_sweep": 0, "pm_init": 1, "cg_iteration": 2, "poisson_coef": 3, "nl_gather": 4, "color": 6}
 
 
@@ -154,6 +155,7 @@ def load_plugins() -> ct.CDLL:
     lib.nli_read_image.argtypes = [ct.c_char_p, _f32p, ct.c_long]
     lib.nli_read_rgb_image.argtypes = [ct.c_char_p, _f32p, ct.c_long]
     lib.nli_read_mono_image.argtypes = [ct.c_char_p, _f32p, ct.c_long]
+    lib.nli_read_pgm_image.argtypes = [ct.c_char_p, _f32p, ct.c_long]
     _plugins = lib
     return lib
 
@@ -751,6 +753,22 @@ def write_mono_image(path: str, img):
     """IOUtility::write_mono_image (io_utility.cpp:192-220): one channel, or the luminance
     0.2126 R + 0.7152 G + 0.0722 B of three."""
     _write(load_plugins().nli_write_mono_image, path, img)
+
+
+def read_pgm_image(path: str):
+    """IOUtility::read_pgm_image (io_utility.cpp:26-74): (H, W) float32 from a P2 or P5
+    file, with the reference's reading rules (include/nli_io.h)."""
+    lib = load_plugins()
+    W, H = ct.c_int(), ct.c_int()
+    _io_check(lib.nli_pgm_info(os.fsencode(path), ct.byref(W), ct.byref(H)))
+    out = np.zeros((H.value, W.value), np.float32)
+    _io_check(lib.nli_read_pgm_image(os.fsencode(path), _ptr(out, _f32p), out.size))
+    return out
+
+
+def write_pgm_image(path: str, img):
+    """IOUtility::write_pgm_image (io_utility.cpp:80-117): P5, one byte per pixel."""
+    _write(load_plugins().nli_write_pgm_image, path, img)
 
 
 def stats_of(st: InpaintStats) -> RunStats:

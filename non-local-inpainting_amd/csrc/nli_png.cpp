@@ -1,8 +1,9 @@
-// PNG files for the inpainting flow: a zlib-only decoder and encoder with the sample
+// Image files for the inpainting flow.  PNG: a zlib-only decoder and encoder with the sample
 // semantics of the reference's image I/O (io_utility.cpp:119-220 over 3rdparty/iio/iio.c,
 // which reads through libpng with PNG_TRANSFORM_EXPAND | PNG_TRANSFORM_PACKING).  The PNG
 // format itself (chunks, CRC-32, the five row filters, Adam7) is the W3C PNG
-// specification.  Host code; exported by libnli_plugins.so.
+// specification.  PGM: the reference's own stdio reader and writer
+// (io_utility.cpp:26-117).  Host code; exported by libnli_plugins.so.
 #include "nli_io.h"
 
 #include <zlib.h>
@@ -384,6 +385,86 @@ int write_bytes(const char *path, const float *img, int W, int H, int C, int CO,
     return encode(path, px, W, H, CO);
 }
 
+// ---- PGM (io_utility.cpp:26-117, 352-377: stdio, no iio) ----
+struct Reader {
+    std::vector<uint8_t> d;
+    size_t at = 0;
+    int get() { return at < d.size() ? d[at++] : EOF; }   // fgetc
+    void unget() { if (at > 0) --at; }
+    static bool space(int c) { return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r'; }
+    // Whitespace runs and '#' comments to the end of their line, as the reference's
+    // skip_spaces_and_comments; the first other character is left unread.
+    void skip_spaces_and_comments()
+    {
+        int c;
+        do {
+            while (space(c = get())) {}
+            if (c == '#')
+                while ((c = get()) != '\n' && c != EOF) {}
+        } while (c == '#');
+        if (c != EOF) unget();
+    }
+    // fscanf("%d"): optional whitespace and sign, then at least one digit.
+    bool scan_int(long &v)
+    {
+        int c;
+        while (space(c = get())) {}
+        bool neg = false;
+        if (c == '+' || c == '-') { neg = c == '-'; c = get(); }
+        if (c < '0' || c > '9') return false;
+        v = 0;
+        for (; c >= '0' && c <= '9'; c = get()) v = std::min(10 * v + (c - '0'), 1L << 40);
+        if (c != EOF) unget();
+        if (neg) v = -v;
+        return true;
+    }
+    // get_number: whitespace, then digits; a non-digit is the reference's exit(1).
+    bool number(int &v)
+    {
+        int c;
+        while (space(c = get())) {}
+        if (c < '0' || c > '9') return false;
+        long n = c - '0';
+        while ((c = get()) >= '0' && c <= '9') n = std::min(10 * n + (c - '0'), 1L << 40);
+        v = (int)n;   // the character after the number is consumed, as there
+        return true;
+    }
+};
+
+struct Pgm { long w = 0, h = 0; bool binary = false; Reader r; };
+
+int pgm_header(const char *path, Pgm &g)
+{
+    if (!path) return fail(NLI_E_ARGUMENT, "path is NULL");
+    if (!read_all(path, g.r.d)) return fail(NLI_E_IO, std::string("cannot read ") + path);
+    Reader &r = g.r;
+    if (r.get() != 'P') return fail(NLI_E_IO, std::string(path) + ": not a PGM file");
+    int c = r.get();
+    if (c != '2' && c != '5') return fail(NLI_E_IO, std::string(path) + ": not a PGM file");
+    g.binary = c == '5';
+    long depth = 0;
+    r.skip_spaces_and_comments();
+    bool ok = r.scan_int(g.w);
+    r.skip_spaces_and_comments();
+    ok = ok && r.scan_int(g.h);
+    r.skip_spaces_and_comments();
+    ok = ok && r.scan_int(depth);
+    if (!ok || g.w <= 0 || g.h <= 0 || g.w * g.h > (1L << 31))
+        return fail(NLI_E_IO, std::string(path) + ": bad PGM header");
+    // The reference skips whitespace and comments before the samples also for binary
+    // data, so leading sample bytes that are whitespace (or '#') are not read as samples.
+    r.skip_spaces_and_comments();
+    return NLI_OK;
+}
+
+// (char)v as x86-64 code converts it: truncation to a 32-bit integer (the
+// "integer indefinite" 0x80000000 for NaN and out-of-range values), low byte kept.
+uint8_t x86_char(double v)
+{
+    int32_t i = (std::isnan(v) || v >= 2147483648.0 || v <= -2147483649.0) ? INT32_MIN : (int32_t)v;
+    return (uint8_t)(uint32_t)i;
+}
+
 // Runs an entry point's body; no exception crosses the C-ABI (a header that declares
 // a huge image can make an allocation fail).
 template <class F> int guarded(F &&body)
@@ -512,6 +593,63 @@ int nli_write_mono_image(const char *path, const float *img, int W, int H, int C
             // io_utility.cpp:212-216: a double expression stored into a float sample.
             return (float)(0.2126 * (double)px[0] + 0.7152 * (double)px[1] + 0.0722 * (double)px[2]);
         });
+    });
+}
+
+int nli_pgm_info(const char *path, int *W, int *H)
+{
+    return guarded([&]() -> int {
+        Pgm g;
+        int rc = pgm_header(path, g);
+        if (rc != NLI_OK) return rc;
+        if (W) *W = (int)g.w;
+        if (H) *H = (int)g.h;
+        return NLI_OK;
+    });
+}
+
+int nli_read_pgm_image(const char *path, float *out, long capacity)
+{
+    return guarded([&]() -> int {
+        if (!out) return fail(NLI_E_ARGUMENT, "out is NULL");
+        Pgm g;
+        int rc = pgm_header(path, g);
+        if (rc != NLI_OK) return rc;
+        if (capacity < g.w * g.h) return fail(NLI_E_ARGUMENT, "output buffer too small");
+        for (long i = 0; i < g.w * g.h; ++i) {
+            int v;
+            if (g.binary) {
+                v = g.r.get();   // past the end: EOF, stored as -1 as the reference stores it
+            } else if (!g.r.number(v)) {
+                return fail(NLI_E_IO, std::string(path) + ": bad PGM sample");
+            }
+            out[i] = (float)v;
+        }
+        return NLI_OK;
+    });
+}
+
+int nli_write_pgm_image(const char *path, const float *img, int W, int H, int C)
+{
+    return guarded([&]() -> int {
+        if (!path || !img) return fail(NLI_E_ARGUMENT, "NULL path or image");
+        if (W <= 0 || H <= 0 || C < 1 || (long long)W * H > (1ll << 31))
+            return fail(NLI_E_ARGUMENT, "image size out of range");
+        std::string o = "P5\n" + std::to_string(W) + " " + std::to_string(H) + "\n255\n";
+        const size_t n = (size_t)W * H;
+        if (C == 1) {
+            for (size_t i = 0; i < n; ++i) o.push_back((char)x86_char((double)img[i]));
+        } else if (C == 3) {
+            for (size_t i = 0; i < n; ++i) {
+                const float *px = img + 3 * i;
+                o.push_back((char)x86_char(0.2126 * px[0] + 0.7152 * px[1] + 0.0722 * px[2]));
+            }
+        }   // other channel counts: the header alone, as the reference writes
+        FILE *f = std::fopen(path, "wb");
+        if (!f) return fail(NLI_E_IO, std::string("cannot write ") + path);
+        bool ok = std::fwrite(o.data(), 1, o.size(), f) == o.size();
+        ok = (std::fclose(f) == 0) && ok;
+        return ok ? NLI_OK : fail(NLI_E_IO, std::string("cannot write ") + path);
     });
 }
 

@@ -29,7 +29,7 @@ def test_plugin_library_exports_its_declared_symbols(nli):
     lib = nli.load_plugins()
     names = declared_symbols("nli_plugins.hpp")
     own = sorted([n for n in names if n.startswith("nli_plain")] + declared_symbols("nli_io.h"))
-    assert len(own) == 11
+    assert len(own) == 14
     assert own == sorted(nli.PLUGIN_EXPORTS)
     missing = [n for n in own if not hasattr(lib, n)]
     assert not missing, missing
