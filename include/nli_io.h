@@ -1,7 +1,7 @@
 /* nli_io.h — image files for the inpainting flow: PNG reading and writing with the
  * sample semantics of the reference's image I/O (io_utility.cpp:119-220, which calls
  * 3rdparty/iio/iio.c).  Host code only, exported by libnli_plugins.so, zlib-based (no
- * libpng).
+ * libpng), and the reference's own stdio PGM reader and writer (io_utility.cpp:26-117).
  *
  * Images are row-major, interleaved (HWC) float32, as everywhere in nli.h.
  * Every function returns NLI_OK (0) or a negative code and never throws.  The message
