@@ -214,6 +214,14 @@ int nli_patchmatch_dev(nli_ctx *ctx, const nli_dist_params *dist, const nli_pm_p
  * counterpart in the reference. */
 int nli_pm_tiles_per_block(int tiles);
 
+/* Row scan of the PatchMatch pool sweep, process-wide: 0 = automatic (the default: on for
+ * NLI_DIST_L2RGBD), 1 = on, 2 = off.  After each sweep's propagation and random search,
+ * every target of a 16-target tile row repeatedly tries the shifted match of its
+ * scan-side neighbour (left on even sweeps, right on odd ones) while that neighbour
+ * improves, as the reference's sequential scan does along a row within one sweep
+ * (patch_match.cpp:197-232).  No counterpart in the reference as a setting. */
+int nli_pm_row_scan(int mode);
+
 /* ---- image updating ---- */
 /* replaces AImageUpdating::update (a_image_updating.h:34-39) of the four updaters:
  * image (W*H*C) is updated in place on the hole pixels (domain); orig is the image the

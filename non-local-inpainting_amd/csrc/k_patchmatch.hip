@@ -88,6 +88,7 @@ struct PMArgs {
     uint64_t seed;
     uint32_t stream;
     const int2 *init;   // initial field (k_pm_init), may be null
+    int row_scan;       // pool sweep: carry improvements along each tile row (see k_pm_sweep_pool)
 };
 
 // Centres whose patch leaves the image are never targets or sources here (the
@@ -596,6 +597,10 @@ __device__ inline void wave_sync()
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
+// candidates are packed as x | y << 16 in 32 unsigned bits (the pool sweep runs only for
+// images below 65536 pixels a side)
+__device__ inline uint32_t pack_xy(int x, int y) { return (uint32_t)x | ((uint32_t)y << 16); }
+__device__ inline int2 unpack_xy(uint32_t v) { return make_int2((int)(v & 0xFFFFu), (int)(v >> 16)); }
 __device__ inline uint64_t pool_key(float d, int order)
 {
     return ((uint64_t)__float_as_uint(d) << 32) | (uint32_t)order;
@@ -632,7 +637,7 @@ struct WavePool {
     uint64_t *keys;   // [kWaveTargets] best (distance, order) of the running pool
     int2 *best;       // [kWaveTargets] current best match
     float *bd;        // [kWaveTargets] its distance (-1: not a target)
-    int *cand;        // [kWaveTargets * tasks]: cx | cy << 16 for task order * 32 + target
+    uint32_t *cand;   // [kWaveTargets * tasks]: pack_xy(cx, cy) for task order * 32 + target
     uint16_t *list;   // [kWaveTargets * tasks]: drawn tasks
 };
 
@@ -674,9 +679,9 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
                 const int j = wp.list[LAST_FIRST ? nvalid - 1 - k : k];
                 t = j % kWaveTargets;
                 order = j / kWaveTargets;
-                const int packed = wp.cand[j];
-                const int cx = packed & 0xFFFF;
-                cy = packed >> 16;
+                const int2 cxy = unpack_xy(wp.cand[j]);
+                const int cx = cxy.x;
+                cy = cxy.y;
                 bound = fminf(wp.bd[t], key_dist(wp.keys[t]));
                 dy = 0;
                 row = a.src + ((size_t)(cy - r) * a.sW + (cx - r)) * C;
@@ -771,7 +776,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
         wp.keys = reinterpret_cast<uint64_t *>(pb);
         wp.best = reinterpret_cast<int2 *>(pb + kWaveTargets * sizeof(uint64_t));
         wp.bd = reinterpret_cast<float *>(pb + kWaveTargets * (sizeof(uint64_t) + sizeof(int2)));
-        wp.cand = reinterpret_cast<int *>(pb + kWaveTargets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)));
+        wp.cand = reinterpret_cast<uint32_t *>(pb + kWaveTargets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)));
         wp.list = reinterpret_cast<uint16_t *>(wp.cand + kWaveTargets * ntask);
     }
 
@@ -839,7 +844,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
                 e.y -= oy;
                 const int2 cur = wp.best[t];
                 if ((e.x != cur.x || e.y != cur.y) && is_source(a, e.x, e.y)) {
-                    wp.cand[j] = e.x | (e.y << 16);
+                    wp.cand[j] = pack_xy(e.x, e.y);
                     ok = true;
                 }
             }
@@ -854,8 +859,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
     if (lw < kWaveTargets) {
         const uint64_t k = wp.keys[lw];
         if (wp.bd[lw] >= 0.0f && key_dist(k) < wp.bd[lw]) {
-            const int packed = wp.cand[(int)(uint32_t)k * kWaveTargets + lw];
-            wp.best[lw] = make_int2(packed & 0xFFFF, packed >> 16);
+            wp.best[lw] = unpack_xy(wp.cand[(int)(uint32_t)k * kWaveTargets + lw]);
             wp.bd[lw] = key_dist(k);
         }
         wp.keys[lw] = kNoKey;
@@ -881,7 +885,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
                     const int cy = y_min + rng.below(y_max - y_min);
                     if (is_source(a, cx, cy)) {
                         ok = cx != ctr.x || cy != ctr.y;
-                        if (ok) wp.cand[j] = cx | (cy << 16);
+                        if (ok) wp.cand[j] = pack_xy(cx, cy);
                         break;
                     }
                 }
@@ -895,19 +899,65 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
     run_pool<KIND, C, P, FP, true, TPB>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);   // smallest windows first
     wave_sync();
     if (lw < kWaveTargets) {
+        const uint64_t k = wp.keys[lw];
+        if (wp.bd[lw] >= 0.0f && key_dist(k) < wp.bd[lw]) {
+            wp.best[lw] = unpack_xy(wp.cand[(int)(uint32_t)k * kWaveTargets + lw]);
+            wp.bd[lw] = key_dist(k);
+        }
+        wp.keys[lw] = kNoKey;
+    }
+    wave_sync();
+
+    // phase 3 (row scan): the reference's sequential scan carries an improvement along a
+    // whole row within one sweep (patch_match.cpp:197-232: each target first tries the
+    // match of its left neighbour, already updated in this sweep; the right one on odd
+    // sweeps).  Within each 16-target tile row of this wave, the targets repeatedly try
+    // the shifted match of their scan-side neighbour whenever that neighbour improved in
+    // the previous round, until no target improves: an improvement travels the whole tile
+    // row in one sweep (the rows lie in one wave, so no workgroup barrier is needed).
+    if (a.row_scan) {
+        const int sh = (a.stream & 1u) ? 1 : -1;   // the neighbour at x + sh proposes its match - sh
+        uint64_t changed = ~0ull;                  // round 0: every neighbour proposes
+        for (int round = 0; round < TX; round++) {
+            bool ok = false;
+            if (lw < kWaveTargets) {
+                const int col = lw % TX + sh, tn = lw + sh;
+                if (col >= 0 && col < TX && ((changed >> tn) & 1ull) && wp.bd[lw] >= 0.0f && wp.bd[tn] >= 0.0f) {
+                    const int2 e = make_int2(wp.best[tn].x - sh, wp.best[tn].y);
+                    const int2 cur = wp.best[lw];
+                    if ((e.x != cur.x || e.y != cur.y) && is_source(a, e.x, e.y)) {
+                        wp.cand[lw] = pack_xy(e.x, e.y);   // task lw: target lw, order 0
+                        ok = true;
+                    }
+                }
+            }
+            const uint64_t m = __ballot(ok);
+            if (m == 0) break;
+            if (ok) wp.list[lane_rank(m)] = (uint16_t)lw;
+            wave_sync();
+            run_pool<KIND, C, P, FP, false, TPB>(a, wl, tiles, LW, tile_floats, wave, wp, __popcll(m));
+            wave_sync();
+            bool improved = false;
+            if (lw < kWaveTargets) {
+                const uint64_t k = wp.keys[lw];
+                if (ok && key_dist(k) < wp.bd[lw]) {
+                    wp.best[lw] = unpack_xy(wp.cand[lw]);
+                    wp.bd[lw] = key_dist(k);
+                    improved = true;
+                }
+                wp.keys[lw] = kNoKey;
+            }
+            changed = __ballot(improved);
+            wave_sync();
+        }
+    }
+
+    if (lw < kWaveTargets) {
         int x, y;
         if (target_xy(lw, x, y)) {
-            const uint64_t k = wp.keys[lw];
-            int2 best = wp.best[lw];
-            float bd = wp.bd[lw];
-            if (key_dist(k) < bd) {
-                const int packed = wp.cand[(int)(uint32_t)k * kWaveTargets + lw];
-                best = make_int2(packed & 0xFFFF, packed >> 16);
-                bd = key_dist(k);
-            }
             const size_t p = (size_t)a.tW * y + x;
-            a.nnf_out[p] = best;
-            a.dist_out[p] = bd;
+            a.nnf_out[p] = wp.best[lw];
+            a.dist_out[p] = wp.bd[lw];
         }
     }
 }
@@ -1074,6 +1124,10 @@ Prepared prepare(const DistDesc &d, const float *src, const uint8_t *smask, int 
     return pr;
 }
 
+// The pool sweep packs source coordinates into 16 bits each and holds at most kMaxR
+// random-search windows.
+bool pool_sweep_fits(int sW, int sH, int R) { return sW <= 65535 && sH <= 65535 && R <= kMaxR; }
+
 int auto_first_step(int W, int H)
 {
     int m = std::max(W, H) / 32;
@@ -1098,6 +1152,7 @@ namespace {
 std::mutex g_tune_mu;
 std::map<std::pair<int, uint64_t>, int> g_tuned;
 std::atomic<int> g_forced_tpb{0};
+std::atomic<int> g_row_scan{0};   // 0: automatic (RGB-D distances), 1: on, 2: off
 
 int current_device()
 {
@@ -1150,6 +1205,21 @@ void set_pm_tiles_per_block(int tpb)
 }
 int pm_tiles_per_block() { return g_forced_tpb.load(); }
 
+void set_pm_row_scan(int mode)
+{
+    if (mode < 0 || mode > 2) throw std::invalid_argument("row scan: 0 (automatic), 1 (on) or 2 (off)");
+    g_row_scan = mode;
+}
+int pm_row_scan() { return g_row_scan.load(); }
+// Automatic: on for L2RGBD, whose depth-gradient term makes the sweep's propagation
+// range matter most; the colour and gradient distances reach the serial reference's
+// energy without it.
+int pm_row_scan_for(int kind)
+{
+    const int m = g_row_scan.load();
+    return m == 1 || (m == 0 && kind == DIST_L2RGBD) ? 1 : 0;
+}
+
 void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const uint8_t *smask, int sW, int sH,
                 const float *tgt, const uint8_t *tmask, int tW, int tH, int C, const int32_t *init_nnf,
                 uint64_t seed, int32_t *out_nnf, float *out_dist, PMWork &w, hipStream_t s)
@@ -1181,6 +1251,7 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     a.nnf_out = n_cur;
     a.dist_out = d_cur;
     a.stream = 0xFFFFu;
+    a.row_scan = pm_row_scan_for(d.kind);
     const int init_r = d.patch / 2;
     const int init_F = (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) ? 3 * C : C;
     const size_t init_lds =
@@ -1205,7 +1276,7 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     const double per_pixel = (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) ? 9.0 * C + 4 : 3.0 * C + 2;
     const double sweep_flops = (double)pr.targets * (2 + R) * d.patch * d.patch * per_pixel;
     // candidate-pool sweep for the specialised gradient distances (up to kMaxR windows)
-    const bool pool = R <= kMaxR && ((d.kind == DIST_L2COMBINED && C == 3 &&
+    const bool pool = pool_sweep_fits(sW, sH, R) && ((d.kind == DIST_L2COMBINED && C == 3 &&
                                       (d.patch == 7 || d.patch == 9 || d.patch == 11)) ||
                                      (d.kind == DIST_L2RGBD && C == 4 && d.patch == 9) ||
                                      ((d.kind == DIST_L2 || d.kind == DIST_L1) && C == 3 &&

@@ -405,6 +405,13 @@ int nli_pm_tiles_per_block(int tiles)
     return NLI_OK;
 }
 
+int nli_pm_row_scan(int mode)
+{
+    if (mode < 0 || mode > 2) return NLI_E_ARGUMENT;
+    nli::set_pm_row_scan(mode);
+    return NLI_OK;
+}
+
 int nli_copy_d2d(nli_ctx *ctx, void *dst, const void *src, uint64_t bytes)
 {
     return guard(ctx, [&] {

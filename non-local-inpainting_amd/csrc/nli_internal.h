@@ -190,6 +190,9 @@ struct PMWork {          // per-call scratch, grown on demand
 // The arrangement does not change results.
 void set_pm_tiles_per_block(int tpb);
 int pm_tiles_per_block();
+void set_pm_row_scan(int mode);
+int pm_row_scan();
+int pm_row_scan_for(int kind);
 
 // The target mask behind a PatchMatch call changed (contents may change behind the
 // same pointer): the cached tile list is rebuilt on the next call.

@@ -9,6 +9,7 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <new>
 #include <cstdint>
@@ -449,7 +450,9 @@ int pgm_header(const char *path, Pgm &g)
     ok = ok && r.scan_int(g.h);
     r.skip_spaces_and_comments();
     ok = ok && r.scan_int(depth);
-    if (!ok || g.w <= 0 || g.h <= 0 || g.w * g.h > (1L << 31))
+    // sides beyond int are refused before the product is formed (it could overflow)
+    if (!ok || g.w <= 0 || g.h <= 0 || g.w > INT_MAX || g.h > INT_MAX ||
+        (uint64_t)g.w * (uint64_t)g.h > (1ull << 31))
         return fail(NLI_E_IO, std::string(path) + ": bad PGM header");
     // The reference skips whitespace and comments before the samples also for binary
     // data, so leading sample bytes that are whitespace (or '#') are not read as samples.

@@ -520,3 +520,15 @@ def test_pgm_writer_equals_reference(nli, tmp_path):
     body = open(str(tmp_path / "mine0.pgm"), "rb").read()[len(b"P5\n7 6\n255\n"):]
     assert list(body[:9]) == [200, 255, 44, 0, 0, 0, 127, 0, 128]
     assert open(str(tmp_path / "mine2.pgm"), "rb").read() == b"P5\n3 3\n255\n"
+
+
+@pytest.mark.parametrize("header", [b"P5\n4294967297 4294967295\n255\n", b"P2\n2147483648 1\n255\n",
+                                    b"P5\n65536 65537\n255\n", b"P5\n1099511627776 2\n255\n"])
+def test_pgm_header_sizes_beyond_int_are_refused(nli, tmp_path, header):
+    """Header sides beyond int, or whose product overflows 64 bits, are refused before
+    any allocation (no image of a wrapped size)."""
+    p = str(tmp_path / "huge.pgm")
+    with open(p, "wb") as f:
+        f.write(header + bytes(16))
+    with pytest.raises(OSError):
+        nli.read_pgm_image(p)

@@ -10,6 +10,8 @@
    masks, FP64 brute-force patch sums, direct sparse solves of the CG system,
    the definition of the weighted median, PatchMatch invariants.
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -491,3 +493,43 @@ def test_process_empty_source_raises(oracle):
     hole = np.ones((24, 24), np.uint8)
     with pytest.raises(RuntimeError, match="Empty source"):
         oracle.process(img, hole, O.InpaintParams(method="nlmeans", patch=7, iters=2, scales=1, coarse=1.0))
+
+
+def test_patchmatch_omp_vs_reference_openmp_build(oracle, syn):
+    """The OpenMP restatement (patch_match.cpp:57-302: chunks of the row-major target list,
+    odd/even buffers, chunk-front copies, the thread-count formula of :149) against the
+    reference's own OpenMP build (oracle/_ref/libnli_ref_omp.so).  The reference seeds
+    each thread's rand_r from the wall clock (:145), so the two are compared by their
+    invariants and by the mean NNF energy over 24 seeds (srand seeds the initial field),
+    within 1 % resolved by the data (ratio +- 2 standard errors)."""
+    if not os.path.exists(O.REF_OMP_SO):
+        pytest.skip("reference OpenMP build not present (make -C oracle ref)")
+    ref = O.Reference(O.REF_OMP_SO)
+    assert ref.lib.ref_is_openmp() == 1
+    W, H = 144, 120
+    img = syn.texture(W, H, 3, 6, 2)
+    hole = syn.ellipse_holes(W, H, 5, 0.15, 2)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 9)
+    t = tgt.astype(bool)
+    dp = oracle.dist_params("l2combined", 9, 10000.0, 0.05)
+    e_o, e_r = [], []
+    for seed in range(1, 25):
+        a = oracle.patchmatch_omp(dp, img, src, img, tgt, None, seed=seed, threads=0)
+        b = ref.patchmatch("l2combined", img, src, img, tgt, None, seed=seed, patch=9, lam=0.05)
+        for f in (a, b):
+            assert np.all(f[~t] == -1)
+            assert np.all(src[f[t][:, 1], f[t][:, 0]] == 1)              # every entry is a source point
+        e_o.append(O.nnf_energy(oracle, dp, img, a, tgt))
+        e_r.append(O.nnf_energy(oracle, dp, img, b, tgt))
+    e_o, e_r = np.array(e_o), np.array(e_r)
+    ratio = e_o.mean() / e_r.mean()
+    se = ratio * np.sqrt(e_o.var(ddof=1) / len(e_o) / e_o.mean() ** 2 + e_r.var(ddof=1) / len(e_r) / e_r.mean() ** 2)
+    print(f"OpenMP restatement / reference OpenMP build energy = {ratio:.4f} (standard error {se:.4f})")
+    assert abs(ratio - 1) + 2 * se <= 0.01
+    # the restatement's own serial limit: one thread is the serial scan of patch_match.cpp:304-512 with
+    # per-thread rand_r draws, so its energy matches the serial reference's
+    s1 = np.array([O.nnf_energy(oracle, dp, img, oracle.patchmatch_omp(dp, img, src, img, tgt, None, seed=s,
+                                                                          threads=1), tgt) for s in range(1, 13)])
+    s0 = np.array([O.nnf_energy(oracle, dp, img, oracle.patchmatch(dp, img, src, img, tgt, None, seed=s), tgt)
+                   for s in range(1, 13)])
+    assert abs(s1.mean() / s0.mean() - 1) <= 0.01
