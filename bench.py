@@ -215,7 +215,11 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--images", type=int, default=8, help="1024x1024 images per step")
-    ap.add_argument("--workers", type=int, default=8, help="concurrent per-image drivers")
+    ap.add_argument("--workers", type=int, default=8,
+                    help="lockstep groups (or, with --batch-mode concurrent, per-image drivers), one stream each")
+    ap.add_argument("--batch-mode", choices=["lockstep", "concurrent"], default="concurrent",
+                    help="lockstep: each group steps its images' scales and iterations together, one launch per "
+                         "kernel for all of them, stopping rule on the device; concurrent: one driver per image")
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the single-image and FP64 runs")
@@ -242,6 +246,7 @@ def main():
 
     nli = importlib.import_module("non-local-inpainting_amd")
     syn = importlib.import_module("non-local-inpainting_amd.synthetic")
+    nli.batch_mode(args.batch_mode)
     if args.workload == "uneven64":
         return bench_uneven(args, nli, syn)
     if args.workload == "rgbd1080":

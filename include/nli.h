@@ -222,6 +222,15 @@ int nli_pm_tiles_per_block(int tiles);
  * (patch_match.cpp:197-232).  No counterpart in the reference as a setting. */
 int nli_pm_row_scan(int mode);
 
+/* How the batch entry points (nli_inpaint_batch*) run, process-wide: 0 (the default) or 2 =
+ * one per-image driver per worker thread, each on its own stream; 1 = in lockstep groups
+ * where the parameters allow it (NL-Poisson / NL-Depth with a candidate-pool PatchMatch,
+ * Poisson or no initialisation, every image with the same parameters): `workers` groups,
+ * each stepping its images' scales and iterations together on one stream, with one launch
+ * per kernel for all of them and the stopping rule applied on the device.  Results do not
+ * depend on the mode.  No counterpart in the reference. */
+int nli_batch_mode(int mode);
+
 /* ---- image updating ---- */
 /* replaces AImageUpdating::update (a_image_updating.h:34-39) of the four updaters:
  * image (W*H*C) is updated in place on the hole pixels (domain); orig is the image the

@@ -44,7 +44,7 @@ EXPORTS = [
     "nli_patchmatch", "nli_patchmatch_dev", "nli_update", "nli_update_dev", "nli_poisson_fill",
     "nli_inpaint", "nli_inpaint_dev", "nli_inpaint_params_default", "nli_profile_enable",
     "nli_patch_energy", "nli_energy_gradient", "nli_patch_extract", "nli_adjoint_aggregate",
-    "nli_pm_tiles_per_block", "nli_pm_row_scan",
+    "nli_pm_tiles_per_block", "nli_pm_row_scan", "nli_batch_mode",
     "nli_profile_read", "nli_inpaint_batch", "nli_inpaint_batch_dev", "nli_inpaint_batch_each",
     "nli_inpaint_batch_each_dev", "nli_rgb_to_lab", "nli_lab_to_rgb", "nli_rgb_to_lab_dev",
     "nli_lab_to_rgb_dev", "nli_inpaint_rgb", "nli_inpaint_rgb_dev", "nli_profile_enable_sampled",
@@ -769,6 +769,14 @@ def read_pgm_image(path: str):
 def write_pgm_image(path: str, img):
     """IOUtility::write_pgm_image (io_utility.cpp:80-117): P5, one byte per pixel."""
     _write(load_plugins().nli_write_pgm_image, path, img)
+
+
+def batch_mode(mode: str):
+    """How batches run, process-wide: "concurrent" (one driver per image; the default) or
+    "lockstep" (groups stepping their images together, where the parameters allow it)."""
+    rc = load_library().nli_batch_mode({"lockstep": 1, "concurrent": 2}[mode])
+    if rc != NLI_OK:
+        raise ValueError(mode)
 
 
 def stats_of(st: InpaintStats) -> RunStats:

@@ -34,6 +34,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <string>
 #include <vector>
 
 namespace nli {
@@ -89,7 +90,27 @@ struct PMArgs {
     uint32_t stream;
     const int2 *init;   // initial field (k_pm_init), may be null
     int row_scan;       // pool sweep: carry improvements along each tile row (see k_pm_sweep_pool)
+    // batched launches (one launch for several images): sweep k reads field k & 1 of the
+    // pair and writes the other, at jump-flood step s0 >> (k mod (levels + 1)); an image
+    // whose *active is 0 is skipped (its stopping rule has ended its iterations)
+    int2 *nnf_pp[2];
+    float *dist_pp[2];
+    int s0, levels;
+    const int *active;
 };
+
+// The image of a block in a batched launch: start[i] is the first block of image i
+// (start[n] = the total), so the image is the last i with start[i] <= b.
+__device__ inline int batch_image(const int *__restrict__ start, int n, int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
 
 // Centres whose patch leaves the image are never targets or sources here (the
 // reference would read outside the image for them; its masks always exclude them).
@@ -364,7 +385,7 @@ __global__ void k_tile_flags(const uint8_t *tmask, int W, int H, int ntx, int nt
 // One thread per target of a 16x4 tile; the tile's target features (with the patch
 // halo) are staged in LDS first, as in the sweeps.
 template <int KIND, int C, int PS>
-__global__ __launch_bounds__(kBlock) void k_pm_init(PMArgs a)
+__device__ void pm_init_body(const PMArgs &a, int tile_slot)
 {
     extern __shared__ float init_lds[];
     constexpr int F = Feat<KIND>::grad ? 3 * C : C;
@@ -373,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_init(PMArgs a)
     const int LW = TX + 2 * hr, LH = TY + 2 * hr;
     const int plane = LW * LH;
     float *wl = init_lds + F * plane;
-    int tile = a.tiles[blockIdx.x];
+    int tile = a.tiles[tile_slot];
     const int x0 = (tile % a.ntiles_x) * TX, y0 = (tile / a.ntiles_x) * TY;
     for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = a.w[i];
     for (int i = threadIdx.x; i < plane; i += blockDim.x) {
@@ -418,6 +439,27 @@ __global__ __launch_bounds__(kBlock) void k_pm_init(PMArgs a)
         a.nnf_out[p] = make_int2(cx, cy);
         a.dist_out[p] = patch_distance<KIND, C, PS>(a, wl, cx, cy, t, FLT_MAX);
     }
+}
+
+template <int KIND, int C, int PS>
+__global__ __launch_bounds__(kBlock) void k_pm_init(PMArgs a)
+{
+    pm_init_body<KIND, C, PS>(a, blockIdx.x);
+}
+
+// batched: one block per target tile of every image; writes field 0 of each image's pair
+template <int KIND, int C, int PS>
+__global__ __launch_bounds__(kBlock) void k_pm_init_batch(const PMArgs *__restrict__ args,
+                                                          const int *__restrict__ start, int nimg)
+{
+    const int b = (int)blockIdx.x;
+    if (b >= start[nimg]) return;
+    const int img = batch_image(start, nimg, b);
+    PMArgs a = args[img];
+    if (a.active && *a.active == 0) return;
+    a.nnf_out = a.nnf_pp[0];
+    a.dist_out = a.dist_pp[0];
+    pm_init_body<KIND, C, PS>(a, b - start[img]);
 }
 
 // (distance, candidate, order) minimum over the 4 lanes of a target group: smallest
@@ -748,7 +790,7 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
 }
 
 template <int KIND, int C, int PS, int TPB>
-__global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
+__device__ void sweep_pool_body(const PMArgs &a, int slot)
 {
     constexpr int kTilesPerBlock = TPB;
     constexpr int kWaveTargets = TPB * TX;
@@ -780,10 +822,6 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
         wp.list = reinterpret_cast<uint16_t *>(wp.cand + kWaveTargets * ntask);
     }
 
-    // block -> pair of tile slots; blocks are dealt to the 8 XCDs round-robin, so each
-    // XCD gets a contiguous band of tiles
-    const int per = (int)(gridDim.x / kXCDs);
-    const int slot = (int)(blockIdx.x % kXCDs) * per + (int)(blockIdx.x / kXCDs);
     if (slot * kTilesPerBlock >= a.ntiles) return;   // whole block: uniform
     int tx0[kTilesPerBlock], ty0[kTilesPerBlock];
     for (int ti = 0; ti < kTilesPerBlock; ti++) {
@@ -960,6 +998,40 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
             a.dist_out[p] = wp.bd[lw];
         }
     }
+}
+
+// Blocks are dealt to the 8 XCDs round-robin: block b is given slot (b mod 8) * per + b / 8,
+// so each XCD gets a contiguous band of tiles and neighbouring targets share its L2.
+__device__ inline int xcd_slot()
+{
+    const int per = (int)(gridDim.x / kXCDs);
+    return (int)(blockIdx.x % kXCDs) * per + (int)(blockIdx.x / kXCDs);
+}
+
+template <int KIND, int C, int PS, int TPB>
+__global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
+{
+    sweep_pool_body<KIND, C, PS, TPB>(a, xcd_slot());
+}
+
+// batched sweep k: start[i] is the first slot (block of TPB tiles) of image i; the slots of
+// all images are dealt to the XCDs in bands as above
+template <int KIND, int C, int PS, int TPB>
+__global__ __launch_bounds__(kBlock) void k_pm_sweep_pool_batch(const PMArgs *__restrict__ args,
+                                                                const int *__restrict__ start, int nimg, int k)
+{
+    const int slot = xcd_slot();
+    if (slot >= start[nimg]) return;
+    const int img = batch_image(start, nimg, slot);
+    PMArgs a = args[img];
+    if (a.active && *a.active == 0) return;
+    a.nnf_in = a.nnf_pp[k & 1];
+    a.dist_in = a.dist_pp[k & 1];
+    a.nnf_out = a.nnf_pp[(k + 1) & 1];
+    a.dist_out = a.dist_pp[(k + 1) & 1];
+    a.step = a.s0 >> (k % (a.levels + 1));
+    a.stream = (uint32_t)k;
+    sweep_pool_body<KIND, C, PS, TPB>(a, slot - start[img]);
 }
 
 template <int KIND, int C, int PS>
@@ -1325,6 +1397,120 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
         prof::end(TK_PM_SWEEP, s, sweep_flops, pr.targets);
         std::swap(n_cur, n_oth);
         std::swap(d_cur, d_oth);
+    }
+}
+
+// ---- several PatchMatch calls in lockstep ----
+bool pm_batch_supported(const DistDesc &d, int C, int sW, int sH, const PMConfig &cfg)
+{
+    const int maxw = (cfg.window != -1) ? cfg.window : std::max(sW, sH);
+    int R = 0;
+    for (int ws = maxw; ws >= 1; ws /= 2) R++;
+    return pool_sweep_fits(sW, sH, R) &&
+           ((d.kind == DIST_L2COMBINED && C == 3 && (d.patch == 7 || d.patch == 9 || d.patch == 11)) ||
+            (d.kind == DIST_L2RGBD && C == 4 && d.patch == 9) ||
+            ((d.kind == DIST_L2 || d.kind == DIST_L1) && C == 3 && (d.patch == 7 || d.patch == 9)));
+}
+
+void patchmatch_batch(const PMJob *jobs, int n, PMBatchWork &bw, hipStream_t s)
+{
+    if (n <= 0) return;
+    const DistDesc &d0 = jobs[0].d;
+    const int C = jobs[0].C, K = std::max(jobs[0].cfg.iterations, 0);
+    for (int i = 0; i < n; i++) {
+        const PMJob &j = jobs[i];
+        if (j.d.kind != d0.kind || j.d.patch != d0.patch || j.C != C || std::max(j.cfg.iterations, 0) != K ||
+            !pm_batch_supported(j.d, j.C, j.sW, j.sH, j.cfg))
+            throw std::invalid_argument("PatchMatch batch: jobs must share a supported distance, channels, patch "
+                                        "and sweep count");
+    }
+    // per job: source gradient and target tiles (prepare), the output and ping-pong
+    // fields reset, the arguments; then one launch per phase for all jobs
+    const size_t args_bytes = sizeof(PMArgs) * n;
+    PMArgs *A = static_cast<PMArgs *>(bw.args.fill(args_bytes));
+    int *st = static_cast<int *>(bw.starts.fill(sizeof(int) * 3 * (n + 1)));
+    int *st_init = st, *st1 = st + (n + 1), *st2 = st + 2 * (n + 1);
+    st_init[0] = st1[0] = st2[0] = 0;
+    int maxR = 0;
+    double sweep_flops = 0, targets = 0;
+    for (int i = 0; i < n; i++) {
+        const PMJob &j = jobs[i];
+        Prepared pr = prepare(j.d, j.src, j.smask, j.sW, j.sH, j.tgt, j.tmask, j.tW, j.tH, C, *j.w, s, true);
+        PMArgs a = pr.a;
+        const size_t tn = (size_t)j.tW * j.tH;
+        float *dA = j.out_dist ? j.out_dist : j.w->dist0.ensure(tn);
+        int32_t *nB = j.w->nnf1.ensure(2 * tn);
+        float *dB = j.w->dist1.ensure(tn);
+        // the final sweep must land in (out_nnf, dA): sweep k reads pair[k & 1]
+        int2 *p0 = (int2 *)((K % 2 == 0) ? j.out_nnf : nB), *p1 = (int2 *)((K % 2 == 0) ? nB : j.out_nnf);
+        float *q0 = (K % 2 == 0) ? dA : dB, *q1 = (K % 2 == 0) ? dB : dA;
+        fill_fields((int32_t *)p0, q0, K > 0 ? (int32_t *)p1 : nullptr, K > 0 ? q1 : nullptr, tn, s);
+        a.shots = j.cfg.shots;
+        a.maxw = (j.cfg.window != -1) ? j.cfg.window : std::max(j.sW, j.sH);
+        a.seed = j.seed;
+        a.init = (const int2 *)j.init_nnf;
+        a.stream = 0xFFFFu;
+        a.row_scan = pm_row_scan_for(j.d.kind);
+        a.nnf_pp[0] = p0;
+        a.nnf_pp[1] = p1;
+        a.dist_pp[0] = q0;
+        a.dist_pp[1] = q1;
+        a.s0 = j.cfg.first_step > 0 ? j.cfg.first_step : auto_first_step(j.tW, j.tH);
+        a.levels = 0;
+        while ((a.s0 >> a.levels) > 1) a.levels++;
+        a.active = j.active;
+        A[i] = a;
+        int R = 0;
+        for (int ws = a.maxw; ws >= 1; ws /= 2) R++;
+        maxR = std::max(maxR, R);
+        st_init[i + 1] = st_init[i] + pr.ntiles;
+        st1[i + 1] = st1[i] + pr.ntiles;
+        st2[i + 1] = st2[i] + (pr.ntiles + 1) / 2;
+        const double per_pixel = (j.d.kind == DIST_L2COMBINED || j.d.kind == DIST_L2RGBD) ? 9.0 * C + 4 : 3.0 * C + 2;
+        sweep_flops += (double)pr.targets * (2 + R) * j.d.patch * j.d.patch * per_pixel;
+        targets += pr.targets;
+    }
+    const PMArgs *dargs = static_cast<const PMArgs *>(bw.args.upload(args_bytes, s));
+    const int *dst = static_cast<const int *>(bw.starts.upload(sizeof(int) * 3 * (n + 1), s));
+    const int total_tiles = st_init[n];
+    if (total_tiles == 0) return;
+    const int P = d0.patch, r = P / 2;
+    const int F = (d0.kind == DIST_L2COMBINED || d0.kind == DIST_L2RGBD) ? 3 * C : C;
+    const size_t init_lds = ((size_t)F * (TX + 2 * r) * (TY + 2 * r) + (size_t)P * P) * sizeof(float);
+    prof::begin(TK_PM_INIT, s);
+#define NLI_INIT_B(KD, CC, PP) k_pm_init_batch<KD, CC, PP><<<total_tiles, TX * TY, init_lds, s>>>(dargs, dst, n)
+    NLI_DISPATCH(d0.kind, C, P, NLI_INIT_B);
+#undef NLI_INIT_B
+    NLI_LAUNCH_CHECK();
+    prof::end(TK_PM_INIT, s, 0.0, targets);
+    auto launch = [&](int tpb, int k) {
+        const int *start = tpb == 1 ? dst + (n + 1) : dst + 2 * (n + 1);
+        const int slots = tpb == 1 ? st1[n] : st2[n];
+        const size_t lds_pool = pool_tables_offset(F, P, tpb) + kWaves * pool_wave_bytes(maxR, tpb);
+        const int blocks = (slots + kXCDs - 1) / kXCDs * kXCDs;
+#define NLI_POOL_B(KD, CC, PP)                                                                        \
+    (tpb == 1 ? k_pm_sweep_pool_batch<KD, CC, PP, 1> : k_pm_sweep_pool_batch<KD, CC, PP, 2>)            \
+        <<<blocks, kBlock, lds_pool, s>>>(dargs, start, n, k)
+        if (d0.kind == DIST_L2RGBD) NLI_POOL_B(DIST_L2RGBD, 4, 9);
+        else if (d0.kind == DIST_L2 && P == 7) NLI_POOL_B(DIST_L2, 3, 7);
+        else if (d0.kind == DIST_L2) NLI_POOL_B(DIST_L2, 3, 9);
+        else if (d0.kind == DIST_L1 && P == 7) NLI_POOL_B(DIST_L1, 3, 7);
+        else if (d0.kind == DIST_L1) NLI_POOL_B(DIST_L1, 3, 9);
+        else if (P == 7) NLI_POOL_B(DIST_L2COMBINED, 3, 7);
+        else if (P == 9) NLI_POOL_B(DIST_L2COMBINED, 3, 9);
+        else NLI_POOL_B(DIST_L2COMBINED, 3, 11);
+#undef NLI_POOL_B
+        NLI_LAUNCH_CHECK();
+    };
+    int tpb = pm_tiles_per_block();
+    const uint64_t tune_key = ((uint64_t)d0.kind << 56) | ((uint64_t)C << 48) | ((uint64_t)P << 40) |
+                              (uint64_t)(32 - __builtin_clz((unsigned)total_tiles | 1u));
+    if (tpb == 0) tpb = tuned_tiles_per_block(tune_key);
+    for (int k = 0; k < K; k++) {
+        if (tpb == 0) tpb = tune_tiles_per_block(tune_key, [&](int t) { launch(t, 0); }, s);
+        prof::begin(TK_PM_SWEEP, s);
+        launch(tpb, k);
+        prof::end(TK_PM_SWEEP, s, sweep_flops, targets);
     }
 }
 

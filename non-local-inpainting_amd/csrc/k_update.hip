@@ -89,11 +89,29 @@ __device__ inline void block_sum(double (&v)[NC])
 }
 
 template <int NC>
-__device__ inline void block_reduce_store(double (&v)[NC], double *partials, int nb)
+__device__ inline void block_reduce_store_at(double (&v)[NC], double *partials, int nb, int blk)
 {
     block_sum<NC>(v);
     if (threadIdx.x == 0)
-        for (int c = 0; c < NC; c++) partials[(size_t)c * nb + blockIdx.x] = v[c];
+        for (int c = 0; c < NC; c++) partials[(size_t)c * nb + blk] = v[c];
+}
+template <int NC>
+__device__ inline void block_reduce_store(double (&v)[NC], double *partials, int nb)
+{
+    block_reduce_store_at<NC>(v, partials, nb, (int)blockIdx.x);
+}
+
+// The job of a block in a batched launch: start[i] is the first block of job i (start[n] =
+// the total), so the job is the last i with start[i] <= b.
+__device__ inline int batch_job(const int *__restrict__ start, int n, int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 // Total of the nb block partials of a one-value reduction, in a fixed order: thread t
@@ -430,6 +448,32 @@ __device__ inline void poisson_coef_rows(int qx, int qy, const float *__restrict
 // neighbour that is itself a hole point is left to its own thread; one that is the
 // neighbour of two hole points is written twice with the same values.
 template <int NC, int CH>
+__device__ inline void poisson_coef_body(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
+                                         const float *__restrict__ img, int Cs, int c0,
+                                         const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
+                                         const float *__restrict__ conf, const float *__restrict__ w, int P,
+                                         float lambda, int W, int H, double *__restrict__ a1,
+                                         double *__restrict__ a2, double *__restrict__ f1, double *__restrict__ F1,
+                                         double *__restrict__ F2, int bx, int row)
+{
+    raise_wave_priority();
+    __shared__ float wl[kMaxP * kMaxP];
+    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
+    __syncthreads();
+    const int k = bx * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int p = pts[k];
+    int x = p % W, y = p / W;
+    if (row == 1) {
+        if (x == 0 || lidx[p - 1] >= 0) return;
+        x -= 1;
+    } else if (row == 2) {
+        if (y == 0 || lidx[p - W] >= 0) return;
+        y -= 1;
+    }
+    poisson_coef_rows<NC, CH>(x, y, img, Cs, c0, ext, nnf, conf, wl, P, lambda, W, H, a1, a2, f1, F1, F2);
+}
+template <int NC, int CH>
 __global__ __launch_bounds__(kBlock) void k_poisson_coef(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
                                const float *__restrict__ img, int Cs, int c0, const uint8_t *__restrict__ ext,
                                const int2 *__restrict__ nnf, const float *__restrict__ conf,
@@ -437,22 +481,8 @@ __global__ __launch_bounds__(kBlock) void k_poisson_coef(const int *__restrict__
                                double *__restrict__ a1, double *__restrict__ a2, double *__restrict__ f1,
                                double *__restrict__ F1, double *__restrict__ F2)
 {
-    raise_wave_priority();
-    __shared__ float wl[kMaxP * kMaxP];
-    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
-    __syncthreads();
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int p = pts[k];
-    int x = p % W, y = p / W;
-    if (blockIdx.y == 1) {
-        if (x == 0 || lidx[p - 1] >= 0) return;
-        x -= 1;
-    } else if (blockIdx.y == 2) {
-        if (y == 0 || lidx[p - W] >= 0) return;
-        y -= 1;
-    }
-    poisson_coef_rows<NC, CH>(x, y, img, Cs, c0, ext, nnf, conf, wl, P, lambda, W, H, a1, a2, f1, F1, F2);
+    poisson_coef_body<NC, CH>(pts, n, lidx, img, Cs, c0, ext, nnf, conf, w, P, lambda, W, H, a1, a2, f1, F1, F2,
+                              (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // The same coefficients with 16 lanes per point (4 points per wavefront): the lanes
@@ -464,14 +494,13 @@ __global__ __launch_bounds__(kBlock) void k_poisson_coef(const int *__restrict__
 // of two per patch row.  Grid row 0: hole points, 1 and 2: their left and top
 // neighbours that are not hole points.
 template <int NC, int SLOTS>
-__global__ __launch_bounds__(64) void k_poisson_coef16(const int *__restrict__ pts, int n,
-                                                       const int *__restrict__ lidx, const float *__restrict__ img,
-                                                       int Cs, int c0, const uint8_t *__restrict__ ext,
-                                                       const int2 *__restrict__ nnf, const float *__restrict__ conf,
-                                                       const float *__restrict__ w, int P, float lambda, int W, int H,
-                                                       double *__restrict__ a1, double *__restrict__ a2,
-                                                       double *__restrict__ f1, double *__restrict__ F1,
-                                                       double *__restrict__ F2)
+__device__ inline void poisson_coef16_body(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
+                                           const float *__restrict__ img, int Cs, int c0,
+                                           const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
+                                           const float *__restrict__ conf, const float *__restrict__ w, int P,
+                                           float lambda, int W, int H, double *__restrict__ a1,
+                                           double *__restrict__ a2, double *__restrict__ f1,
+                                           double *__restrict__ F1, double *__restrict__ F2, int bx, int row)
 {
     raise_wave_priority();
     static_assert(2 + 3 * NC <= kNLG, "one lane per accumulator");
@@ -482,17 +511,17 @@ __global__ __launch_bounds__(64) void k_poisson_coef16(const int *__restrict__ p
     for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
     __syncthreads();
     const int lane = (int)threadIdx.x, gl = lane % kNLG, grp = lane / kNLG;
-    const int k = blockIdx.x * kNLPix + grp;
+    const int k = bx * kNLPix + grp;
     bool live = k < n;
     int qx = 0, qy = 0;
     if (live) {
         const int p = pts[k];
         qx = p % W;
         qy = p / W;
-        if (blockIdx.y == 1) {
+        if (row == 1) {
             live = qx > 0 && lidx[p - 1] < 0;
             qx -= 1;
-        } else if (blockIdx.y == 2) {
+        } else if (row == 2) {
             live = qy > 0 && lidx[p - W] < 0;
             qy -= 1;
         }
@@ -570,6 +599,19 @@ __global__ __launch_bounds__(64) void k_poisson_coef16(const int *__restrict__ p
     else if (gl < 2 + 2 * NC) F1[(gl - 2 - NC) * plane + q] = acc;
     else F2[(gl - 2 - 2 * NC) * plane + q] = acc;
 }
+template <int NC, int SLOTS>
+__global__ __launch_bounds__(64) void k_poisson_coef16(const int *__restrict__ pts, int n,
+                                                       const int *__restrict__ lidx, const float *__restrict__ img,
+                                                       int Cs, int c0, const uint8_t *__restrict__ ext,
+                                                       const int2 *__restrict__ nnf, const float *__restrict__ conf,
+                                                       const float *__restrict__ w, int P, float lambda, int W, int H,
+                                                       double *__restrict__ a1, double *__restrict__ a2,
+                                                       double *__restrict__ f1, double *__restrict__ F1,
+                                                       double *__restrict__ F2)
+{
+    poisson_coef16_body<NC, SLOTS>(pts, n, lidx, img, Cs, c0, ext, nnf, conf, w, P, lambda, W, H, a1, a2, f1, F1,
+                                   F2, (int)blockIdx.x, (int)blockIdx.y);
+}
 
 template <int NC>
 void launch_poisson_coef(int P, dim3 grid, hipStream_t s, const int *pts, int n, const int *lidx, const float *img,
@@ -637,17 +679,17 @@ __device__ inline double lap(const Stencil &s, const int4 &nb, const double *v, 
 // stencil and neighbour indices (reference coefficient choice: top/left use a2 of that
 // neighbour, bottom/right a2 of the centre; 0 outside the image), the right-hand side
 // b = f1 - f2 - a1 u0 + L(u0) with u0 over the full image, and x0 = image - u0.
-__global__ void k_cg_prepare(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
-                             const float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0, int nc,
-                             const double *__restrict__ a1, const double *__restrict__ a2,
-                             const double *__restrict__ f1, const double *__restrict__ F1,
-                             const double *__restrict__ F2, int div_active, int W, int H,
-                             Stencil *__restrict__ st, int4 *__restrict__ nbr, double *__restrict__ b,
-                             double *__restrict__ x0, int *__restrict__ done_at)
+__device__ inline void cg_prepare_body(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
+                                       const float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0,
+                                       int nc, const double *__restrict__ a1, const double *__restrict__ a2,
+                                       const double *__restrict__ f1, const double *__restrict__ F1,
+                                       const double *__restrict__ F2, int div_active, int W, int H,
+                                       Stencil *__restrict__ st, int4 *__restrict__ nbr, double *__restrict__ b,
+                                       double *__restrict__ x0, int *__restrict__ done_at, int blk)
 {
     raise_wave_priority();
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x < kMaxNC) done_at[threadIdx.x] = -1;   // the CG's stop flags
+    const int k = blk * blockDim.x + threadIdx.x;
+    if (blk == 0 && threadIdx.x < kMaxNC) done_at[threadIdx.x] = -1;   // the CG's stop flags
     if (k >= n) return;
     const int p = pts[k];
     const int x = p % W, y = p / W;
@@ -684,6 +726,17 @@ __global__ void k_cg_prepare(const int *__restrict__ pts, int n, const int *__re
         x0[(size_t)ch * n + k] = (double)img[(size_t)p * Cs + c0 + ch] - u0;
     }
 }
+__global__ void k_cg_prepare(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
+                             const float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0, int nc,
+                             const double *__restrict__ a1, const double *__restrict__ a2,
+                             const double *__restrict__ f1, const double *__restrict__ F1,
+                             const double *__restrict__ F2, int div_active, int W, int H,
+                             Stencil *__restrict__ st, int4 *__restrict__ nbr, double *__restrict__ b,
+                             double *__restrict__ x0, int *__restrict__ done_at)
+{
+    cg_prepare_body(pts, n, lidx, img, orig, Cs, c0, nc, a1, a2, f1, F1, F2, div_active, W, H, st, nbr, b, x0,
+                    done_at, (int)blockIdx.x);
+}
 
 // ---------------- CG ----------------
 // Device state per channel c:
@@ -713,6 +766,10 @@ struct CGArgs {
     float tol;
     int limit;
     int refine_every;  // T = float: residual recomputed in FP64 every k iterations (0 = never)
+    // batched launches: the FP64 initial guess, and the image's flag (0: its stopping rule
+    // ended its iterations; the kernels then only announce its polls as finished)
+    const double *x0d;
+    const int *active;
 };
 
 // Grid totals of the previous kernel's block partials part[c * nb + block]; every
@@ -746,10 +803,10 @@ struct StencilOf {
 
 // r0 = b - A x0 (FP64), d0 = r0, partials of |r0|^2
 template <int NC, typename T>
-__global__ void k_cg_init(CGArgs<T> a, const double *x0)
+__device__ inline void cg_init_body(const CGArgs<T> &a, const double *x0, int blk)
 {
     raise_wave_priority();
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blk * blockDim.x + threadIdx.x;
     double acc[NC];
     for (int c = 0; c < NC; c++) acc[c] = 0.0;
     if (k < a.n) {
@@ -766,7 +823,12 @@ __global__ void k_cg_init(CGArgs<T> a, const double *x0)
             acc[c] = rv * rv;
         }
     }
-    block_reduce_store<NC>(acc, a.prr, a.nb);
+    block_reduce_store_at<NC>(acc, a.prr, a.nb, blk);
+}
+template <int NC, typename T>
+__global__ void k_cg_init(CGArgs<T> a, const double *x0)
+{
+    cg_init_body<NC, T>(a, x0, (int)blockIdx.x);
 }
 
 // The per-channel scalars of an iteration (the stop test on |r_it|^2, beta, alpha, and
@@ -799,10 +861,10 @@ struct Gather5 {
 // phase A of iteration it: stop test on |r_it|^2, d_it = r_it + beta d_{it-1} (it > 0),
 // partials of <r_it, A d_it>
 template <int NC, typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_a(CGArgs<T> a, int it)
+__device__ inline void cg_a_body(const CGArgs<T> &a, int it, int blk)
 {
     raise_wave_priority();
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blk * blockDim.x + threadIdx.x;
     const bool live = k < a.n;
     const T *dp = a.d[it & 1];
     T *dn = a.d[(it + 1) & 1];
@@ -835,7 +897,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_a(CGArgs<T> a, int it)
             const double nr = tot[c];
             const double nr0 = (it == 0) ? nr : h0[c];
             const bool stop = (it == 0) ? !(nr0 > 0) : ((nr < (double)a.tol * nr0) || (it - 1 >= a.limit));
-            if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (blk == 0 && threadIdx.x == 0) {
                 a.hist[(size_t)c * a.hist_len + it] = nr;
                 if (stop) a.done_at[c] = it;
             }
@@ -846,7 +908,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_a(CGArgs<T> a, int it)
     }
     // the stop state for the host's convergence poll, written straight to pinned host
     // memory (no copy in the stream); slot parity follows the poll count
-    if ((it + 1) % a.poll_every == 0 && blockIdx.x == 0 && threadIdx.x == 0 && a.poll_host) {
+    if ((it + 1) % a.poll_every == 0 && blk == 0 && threadIdx.x == 0 && a.poll_host) {
         const int chunk = (it + 1) / a.poll_every - 1;
         int *slot = a.poll_host + (chunk & 1) * kPollSlot;
 #pragma unroll
@@ -874,16 +936,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_a(CGArgs<T> a, int it)
             acc[c] = (double)rk * (double)(s.a1 * D[0] - L);
         }
     }
-    block_reduce_store<NC>(acc, a.prq, a.nb);
+    block_reduce_store_at<NC>(acc, a.prq, a.nb, blk);
+}
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_a(CGArgs<T> a, int it)
+{
+    cg_a_body<NC, T>(a, it, (int)blockIdx.x);
 }
 
 // phase B of iteration it: alpha = |r_it|^2 / <r_it, A d_it>, x_{it+1} = x_it + alpha d_it,
 // r_{it+1} = b - A x_{it+1}, partials of |r_{it+1}|^2
 template <int NC, typename T>
-__global__ __launch_bounds__(kBlock) void k_cg_b(CGArgs<T> a, int it)
+__device__ inline void cg_b_body(const CGArgs<T> &a, int it, int blk)
 {
     raise_wave_priority();
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blk * blockDim.x + threadIdx.x;
     const bool live = k < a.n;
     const T *xp = a.x[it & 1];
     T *xn = a.x[(it + 1) & 1];
@@ -942,17 +1009,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_b(CGArgs<T> a, int it)
             acc[c] = rv * rv;
         }
     }
-    block_reduce_store<NC>(acc, a.prr, a.nb);
+    block_reduce_store_at<NC>(acc, a.prr, a.nb, blk);
+}
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_b(CGArgs<T> a, int it)
+{
+    cg_b_body<NC, T>(a, it, (int)blockIdx.x);
 }
 
 // final colours: image = float(u0 + x_final) on hole points; sum of (prev - new)^2
 template <typename T>
-__global__ void k_poisson_write(float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0, int nc,
-                                const int *__restrict__ pts, int n, T *x0, T *x1, const int *__restrict__ done_at,
-                                double *__restrict__ partials)
+__device__ inline void poisson_write_body(float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0,
+                                          int nc, const int *__restrict__ pts, int n, const T *x0, const T *x1,
+                                          const int *__restrict__ done_at, double *__restrict__ partials, int nb,
+                                          int blk)
 {
     raise_wave_priority();
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = blk * blockDim.x + threadIdx.x;
     double acc[1] = {0.0};
     if (k < n) {
         size_t p = (size_t)pts[k];
@@ -966,7 +1039,15 @@ __global__ void k_poisson_write(float *__restrict__ img, const float *__restrict
             img[p * Cs + c0 + ch] = v;
         }
     }
-    block_reduce_store<1>(acc, partials, (int)gridDim.x);
+    block_reduce_store_at<1>(acc, partials, nb, blk);
+}
+template <typename T>
+__global__ void k_poisson_write(float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0, int nc,
+                                const int *__restrict__ pts, int n, T *x0, T *x1, const int *__restrict__ done_at,
+                                double *__restrict__ partials)
+{
+    poisson_write_body<T>(img, orig, Cs, c0, nc, pts, n, x0, x1, done_at, partials, (int)gridDim.x,
+                          (int)blockIdx.x);
 }
 
 inline unsigned grid_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -1276,6 +1357,403 @@ double update(int kind, int patch, float sigma, float lambda, const CGParams &cg
         return wait_summary(w, sa, s)[kSumTotal];
     }
     throw std::invalid_argument("unknown image updating kind");
+}
+
+}  // namespace nli
+
+// ======================= updates in lockstep =======================
+// One launch per kernel for several NL-Poisson updates: every kernel of poisson() above,
+// with each block belonging to one job (start[] tables), running the same device bodies on
+// the job's own buffers, so that each job's results equal a single update's.  A job whose
+// image has stopped iterating (*active == 0) is skipped; its CG polls are announced as
+// finished.  The last kernel applies the inpainting loop's stopping rule on the device.
+namespace nli {
+namespace {
+
+template <typename T>
+struct PoissonJob {
+    const int *active;
+    const int *pts;
+    int n;
+    const int *lidx;
+    float *img;
+    const float *orig;
+    int Cs, c0, nc;
+    const uint8_t *ext;
+    const int2 *nnf;
+    const float *conf;
+    const float *w;
+    int P;
+    float lambda;
+    int W, H;
+    double *a1, *a2, *f1, *F1, *F2;
+    Stencil *st;
+    int4 *nbr;
+    double *b, *x0;
+    int *done_at;
+    int div_active;
+    CGArgs<T> cg;
+    double *wpart;     // block partials of the total change
+    double *scalars;   // the update's summary (kSummary doubles)
+    StopRule stop;
+};
+
+__device__ inline bool job_active(const int *active) { return !active || *active != 0; }
+
+template <int NC, int CH, typename T>
+__global__ __launch_bounds__(kBlock) void k_poisson_coef_batch(const PoissonJob<T> *__restrict__ jobs,
+                                                               const int *__restrict__ start, int n)
+{
+    const int b = (int)blockIdx.x;
+    if (b >= start[n]) return;
+    const int j = batch_job(start, n, b);
+    const PoissonJob<T> &J = jobs[j];
+    if (!job_active(J.active)) return;
+    const int lb = b - start[j], nbj = (J.n + kBlock - 1) / kBlock;
+    poisson_coef_body<NC, CH>(J.pts, J.n, J.lidx, J.img, J.Cs, J.c0, J.ext, J.nnf, J.conf, J.w, J.P, J.lambda, J.W,
+                              J.H, J.a1, J.a2, J.f1, J.F1, J.F2, lb % nbj, lb / nbj);
+}
+
+template <int NC, int SLOTS, typename T>
+__global__ __launch_bounds__(64) void k_poisson_coef16_batch(const PoissonJob<T> *__restrict__ jobs,
+                                                             const int *__restrict__ start, int n)
+{
+    const int b = (int)blockIdx.x;
+    if (b >= start[n]) return;
+    const int j = batch_job(start, n, b);
+    const PoissonJob<T> &J = jobs[j];
+    if (!job_active(J.active)) return;
+    const int lb = b - start[j], nbj = (J.n + kNLPix - 1) / kNLPix;
+    poisson_coef16_body<NC, SLOTS>(J.pts, J.n, J.lidx, J.img, J.Cs, J.c0, J.ext, J.nnf, J.conf, J.w, J.P,
+                                   J.lambda, J.W, J.H, J.a1, J.a2, J.f1, J.F1, J.F2, lb % nbj, lb / nbj);
+}
+
+template <typename T>
+__global__ void k_cg_prepare_batch(const PoissonJob<T> *__restrict__ jobs, const int *__restrict__ start, int n)
+{
+    const int b = (int)blockIdx.x;
+    if (b >= start[n]) return;
+    const int j = batch_job(start, n, b);
+    const PoissonJob<T> &J = jobs[j];
+    if (!job_active(J.active)) return;
+    cg_prepare_body(J.pts, J.n, J.lidx, J.img, J.orig, J.Cs, J.c0, J.nc, J.a1, J.a2, J.f1, J.F1, J.F2, J.div_active,
+                    J.W, J.H, J.st, J.nbr, J.b, J.x0, J.done_at, b - start[j]);
+}
+
+template <int NC, typename T>
+__global__ void k_cg_init_batch(const PoissonJob<T> *__restrict__ jobs, const int *__restrict__ start, int n)
+{
+    const int b = (int)blockIdx.x;
+    if (b >= start[n]) return;
+    const int j = batch_job(start, n, b);
+    const PoissonJob<T> &J = jobs[j];
+    if (!job_active(J.active)) return;
+    cg_init_body<NC, T>(J.cg, J.cg.x0d, b - start[j]);
+}
+
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_a_batch(const PoissonJob<T> *__restrict__ jobs,
+                                                       const int *__restrict__ start, int n, int it)
+{
+    const int b = (int)blockIdx.x;
+    if (b >= start[n]) return;
+    const int j = batch_job(start, n, b);
+    const PoissonJob<T> &J = jobs[j];
+    const int lb = b - start[j];
+    if (!job_active(J.active)) {
+        // the image has stopped iterating: its polls report every channel finished
+        const CGArgs<T> &a = J.cg;
+        if (lb == 0 && threadIdx.x == 0 && a.poll_host && (it + 1) % a.poll_every == 0) {
+            const int chunk = (it + 1) / a.poll_every - 1;
+            int *slot = a.poll_host + (chunk & 1) * kPollSlot;
+            for (int c = 0; c < kMaxNC; c++) slot[c] = 0;
+            __threadfence_system();
+            reinterpret_cast<volatile unsigned *>(slot)[kMaxNC] = a.poll_base + (unsigned)chunk;
+        }
+        return;
+    }
+    cg_a_body<NC, T>(J.cg, it, lb);
+}
+
+template <int NC, typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_b_batch(const PoissonJob<T> *__restrict__ jobs,
+                                                       const int *__restrict__ start, int n, int it)
+{
+    const int b = (int)blockIdx.x;
+    if (b >= start[n]) return;
+    const int j = batch_job(start, n, b);
+    const PoissonJob<T> &J = jobs[j];
+    if (!job_active(J.active)) return;
+    cg_b_body<NC, T>(J.cg, it, b - start[j]);
+}
+
+template <typename T>
+__global__ void k_poisson_write_batch(const PoissonJob<T> *__restrict__ jobs, const int *__restrict__ start, int n)
+{
+    const int b = (int)blockIdx.x;
+    if (b >= start[n]) return;
+    const int j = batch_job(start, n, b);
+    const PoissonJob<T> &J = jobs[j];
+    if (!job_active(J.active)) return;
+    poisson_write_body<T>(J.img, J.orig, J.Cs, J.c0, J.nc, J.pts, J.n, J.cg.x[0], J.cg.x[1], J.done_at, J.wpart,
+                          J.cg.nb, b - start[j]);
+}
+
+// Block j: job j's total change (the partials summed as k_sum_partials sums them), its CG
+// summary, and the stopping rule.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sum_batch(const PoissonJob<T> *__restrict__ jobs)
+{
+    raise_wave_priority();
+    const PoissonJob<T> &J = jobs[blockIdx.x];
+    const StopRule &R = J.stop;
+    if (!job_active(J.active)) {
+        if (R.host && threadIdx.x == 0) {
+            reinterpret_cast<volatile int *>(R.host)[0] = 0;
+            __threadfence_system();
+            reinterpret_cast<volatile unsigned *>(R.host)[1] = R.stamp;
+        }
+        return;
+    }
+    __shared__ double sh[kBlock];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < J.cg.nb; b += kBlock) s += J.wpart[b];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double total = sh[0];
+    J.scalars[kSumTotal] = total;
+    int cg_it = 0;
+    for (int c = 0; c < J.nc && J.n > 0; c++) {
+        const int d = J.done_at[c];
+        const double *h = J.cg.hist + (size_t)c * J.cg.hist_len;
+        J.scalars[kSumDone + c] = (double)d;
+        J.scalars[kSumNr0 + c] = h[0];
+        J.scalars[kSumNr + c] = h[d > 0 ? d : 0];
+        cg_it += d;
+    }
+    if (R.cg_total) *R.cg_total += cg_it;
+    if (!R.active) return;
+    if (R.part == 1) {
+        *R.partial = total;
+        return;
+    }
+    const double t = R.part == 2 ? *R.partial + total : total;
+    *R.total = t;
+    const int cnt = *R.count + 1;
+    *R.count = cnt;
+    const int act = (t > (double)R.tol && cnt < R.iters) ? 1 : 0;
+    *R.active = act;
+    if (R.host) {
+        reinterpret_cast<volatile int *>(R.host)[0] = act;
+        __threadfence_system();
+        reinterpret_cast<volatile unsigned *>(R.host)[1] = R.stamp;
+    }
+}
+
+template <int NC, typename T>
+void poisson_batch(const UpdJob *jobs, int n, UpdBatchWork &bw, hipStream_t s)
+{
+    const int P = jobs[0].patch;
+    const CGParams &cg0 = jobs[0].cg;
+    const size_t jb = sizeof(PoissonJob<T>) * n;
+    PoissonJob<T> *J = static_cast<PoissonJob<T> *>(bw.jobs.fill(jb));
+    int *st = static_cast<int *>(bw.starts.fill(sizeof(int) * 3 * (n + 1)));
+    int *st_coef = st, *st_c16 = st + (n + 1), *st_nb = st + 2 * (n + 1);
+    st_coef[0] = st_c16[0] = st_nb[0] = 0;
+    const int poll = 2;
+    const unsigned poll_base = bw.poll_seq;
+    bw.poll_seq += (unsigned)((cg0.limit + 2) / poll + 2);
+    for (int i = 0; i < n; i++) {
+        const UpdJob &u = jobs[i];
+        UpdWork &w = *u.w;
+        const size_t np = (size_t)u.W * u.H;
+        const int m = u.n_hole;
+        upload_weights(w, u.patch, u.sigma, s);
+        w.a1.ensure(np);
+        w.a2.ensure(np);
+        w.f1.ensure(np * NC);
+        w.F1.ensure(np * NC);
+        w.F2.ensure(np * NC);
+        const int nb = (int)grid_for(std::max(m, 1));
+        w.scalars.ensure(kSummary + (size_t)kMaxNC * (u.cg.limit + 2));
+        if (!w.done_host) {
+            NLI_HIP(hipHostMalloc((void **)&w.done_host, sizeof(int) * 2 * kPollSlot,
+                                  hipHostMallocCoherent | hipHostMallocMapped));
+            std::memset(w.done_host, 0, sizeof(int) * 2 * kPollSlot);
+            NLI_HIP(hipHostGetDevicePointer((void **)&w.done_dev, w.done_host, 0));
+        }
+        w.partials.ensure((size_t)2 * kMaxNC * nb + 8);
+        w.flag.ensure(2 + kMaxNC + 2);
+        w.lidx.ensure(np);
+        if (m > 0 && (w.lidx_version != w.pts_version || w.lidx_of != w.lidx.p || w.lidx_pts != u.hole_pts ||
+                      w.lidx_n != m || w.lidx_np != np)) {
+            fill_i32(w.lidx.p, -1, np, s);
+            k_lidx<<<grid_for(m), kBlock, 0, s>>>(u.hole_pts, m, w.lidx.p);
+            NLI_LAUNCH_CHECK();
+            w.lidx_version = w.pts_version;
+            w.lidx_of = w.lidx.p;
+            w.lidx_pts = u.hole_pts;
+            w.lidx_n = m;
+            w.lidx_np = np;
+        }
+        w.stencil.ensure((size_t)std::max(m, 1) * (sizeof(Stencil) / sizeof(double)));
+        w.nbr.ensure((size_t)std::max(m, 1) * 4);
+        w.b.ensure((size_t)std::max(m, 1) * NC);
+        w.x0.ensure((size_t)std::max(m, 1) * NC);
+        CGArgs<T> a;
+        const size_t vn = (size_t)std::max(m, 1) * NC;
+        if (std::is_same<T, double>::value) {
+            w.x1.ensure(vn);
+            w.d0.ensure(vn);
+            w.d1.ensure(vn);
+            w.r.ensure(vn);
+            a = cg_args<T>(u.cg, m, NC, w, (Stencil *)w.stencil.p, (int4 *)w.nbr.p, (T *)w.x0.p, (T *)w.x1.p,
+                           (T *)w.d0.p, (T *)w.d1.p, (T *)w.r.p);
+        } else {
+            w.xs0.ensure(vn);
+            w.xs1.ensure(vn);
+            w.ds0.ensure(vn);
+            w.ds1.ensure(vn);
+            w.rs.ensure(vn);
+            a = cg_args<T>(u.cg, m, NC, w, (Stencil *)w.stencil.p, (int4 *)w.nbr.p, (T *)w.xs0.p, (T *)w.xs1.p,
+                           (T *)w.ds0.p, (T *)w.ds1.p, (T *)w.rs.p);
+        }
+        a.poll_base = poll_base;
+        a.poll_every = poll;
+        a.x0d = w.x0.p;
+        a.active = u.active;
+        PoissonJob<T> &q = J[i];
+        q.active = u.active;
+        q.pts = u.hole_pts;
+        q.n = m;
+        q.lidx = w.lidx.p;
+        q.img = u.image;
+        q.orig = u.orig;
+        q.Cs = u.Cs;
+        q.c0 = u.c0;
+        q.nc = NC;
+        q.ext = u.ext;
+        q.nnf = (const int2 *)u.nnf;
+        q.conf = u.conf;
+        q.w = w.weights.p;
+        q.P = u.patch;
+        q.lambda = u.lambda;
+        q.W = u.W;
+        q.H = u.H;
+        q.a1 = w.a1.p;
+        q.a2 = w.a2.p;
+        q.f1 = w.f1.p;
+        q.F1 = w.F1.p;
+        q.F2 = w.F2.p;
+        q.st = (Stencil *)w.stencil.p;
+        q.nbr = (int4 *)w.nbr.p;
+        q.b = w.b.p;
+        q.x0 = w.x0.p;
+        q.done_at = w.flag.p + 2;
+        q.div_active = u.lambda != 1 ? 1 : 0;
+        q.cg = a;
+        q.wpart = w.partials.p;
+        q.scalars = w.scalars.p;
+        q.stop = u.stop;
+        const bool c16 = m < 32768;
+        st_coef[i + 1] = st_coef[i] + (m > 0 && !c16 ? 3 * (int)grid_for(m) : 0);
+        st_c16[i + 1] = st_c16[i] + (m > 0 && c16 ? 3 * ((m + kNLPix - 1) / kNLPix) : 0);
+        st_nb[i + 1] = st_nb[i] + (m > 0 ? (int)grid_for(m) : 0);
+    }
+    const PoissonJob<T> *dj = static_cast<const PoissonJob<T> *>(bw.jobs.upload(jb, s));
+    const int *ds = static_cast<const int *>(bw.starts.upload(sizeof(int) * 3 * (n + 1), s));
+    const int *d_coef = ds, *d_c16 = ds + (n + 1), *d_nb = ds + 2 * (n + 1);
+    prof::begin(TK_POISSON_COEF, s);
+    if (st_c16[n] > 0) {
+#define NLI_C16B(S) k_poisson_coef16_batch<NC, S, T><<<st_c16[n], 64, 0, s>>>(dj, d_c16, n)
+        const int pp = P * P;
+        if (pp <= 32) NLI_C16B(2);
+        else if (pp <= 64) NLI_C16B(4);
+        else if (pp <= 96) NLI_C16B(6);
+        else if (pp <= 128) NLI_C16B(8);
+        else NLI_C16B(15);
+#undef NLI_C16B
+    }
+    if (st_coef[n] > 0) {
+#define NLI_CB(CH) k_poisson_coef_batch<NC, CH, T><<<st_coef[n], kBlock, 0, s>>>(dj, d_coef, n)
+        switch (P) {
+        case 5: NLI_CB(5); break;
+        case 7: NLI_CB(7); break;
+        case 9: NLI_CB(9); break;
+        case 11: NLI_CB(11); break;
+        default: NLI_CB(4); break;
+        }
+#undef NLI_CB
+    }
+    NLI_LAUNCH_CHECK();
+    prof::end(TK_POISSON_COEF, s, 0.0, 0.0);
+    const int nbt = st_nb[n];
+    if (nbt > 0) {
+        k_cg_prepare_batch<T><<<nbt, kBlock, 0, s>>>(dj, d_nb, n);
+        k_cg_init_batch<NC, T><<<nbt, kBlock, 0, s>>>(dj, d_nb, n);
+        NLI_LAUNCH_CHECK();
+        double it_bytes = 0;
+        for (int i = 0; i < n; i++) it_bytes += 2.0 * 13 * sizeof(T) * NC * (double)jobs[i].n_hole;
+        int chunk = 0;
+        for (int it = 0; it <= cg0.limit + 1; it++) {
+            prof::begin(TK_CG, s);
+            k_cg_a_batch<NC, T><<<nbt, kBlock, 0, s>>>(dj, d_nb, n, it);
+            if (it <= cg0.limit) k_cg_b_batch<NC, T><<<nbt, kBlock, 0, s>>>(dj, d_nb, n, it);
+            NLI_LAUNCH_CHECK();
+            prof::end(TK_CG, s, it_bytes, 0.0);
+            if ((it + 1) % poll == 0 && it <= cg0.limit) {
+                if (chunk > 0) {
+                    bool all = true;
+                    for (int i = 0; i < n && all; i++) {
+                        if (jobs[i].n_hole == 0) continue;
+                        const volatile int *slot =
+                            wait_poll(*jobs[i].w, (chunk - 1) & 1, poll_base + (unsigned)(chunk - 1), s);
+                        for (int c = 0; c < NC; c++) all = all && slot[c] >= 0;
+                    }
+                    if (all) break;
+                }
+                chunk++;
+            }
+        }
+        k_poisson_write_batch<T><<<nbt, kBlock, 0, s>>>(dj, d_nb, n);
+        NLI_LAUNCH_CHECK();
+    }
+    k_sum_batch<T><<<n, kBlock, 0, s>>>(dj);
+    NLI_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void update_batch(const UpdJob *jobs, int n, UpdBatchWork &bw, hipStream_t s)
+{
+    if (n <= 0) return;
+    const UpdJob &j0 = jobs[0];
+    for (int i = 0; i < n; i++) {
+        const UpdJob &u = jobs[i];
+        if (u.kind != j0.kind || u.patch != j0.patch || u.nc != j0.nc || u.cg.precision != j0.cg.precision ||
+            u.cg.limit != j0.cg.limit)
+            throw std::invalid_argument("update batch: jobs must share the kind, patch, channels and CG settings");
+        if (u.patch < 1 || u.patch > kMaxP) throw std::invalid_argument("patch side must be in [1, 15]");
+        if (u.nc < 1 || u.nc > kMaxNC || u.c0 < 0 || u.c0 + u.nc > u.Cs)
+            throw std::invalid_argument("bad channel range");
+    }
+    if (j0.kind != UPD_POISSON) throw std::invalid_argument("update batch: NL-Poisson updates only");
+    auto run = [&](auto nc_tag) {
+        constexpr int NC = decltype(nc_tag)::value;
+        if (j0.cg.precision == 0) poisson_batch<NC, double>(jobs, n, bw, s);
+        else poisson_batch<NC, float>(jobs, n, bw, s);
+    };
+    switch (j0.nc) {
+    case 1: run(std::integral_constant<int, 1>{}); break;
+    case 2: run(std::integral_constant<int, 2>{}); break;
+    case 3: run(std::integral_constant<int, 3>{}); break;
+    default: run(std::integral_constant<int, 4>{}); break;
+    }
 }
 
 }  // namespace nli

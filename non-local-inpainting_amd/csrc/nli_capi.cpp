@@ -28,6 +28,9 @@ struct nli_worker {
     nli::DriverWork drv;
     nli::DBuf<float> fin, fout;
     nli::DBuf<uint8_t> min;
+    nli::LockstepWork lock;                                     // lockstep groups
+    std::vector<nli::DBuf<float>> lin, lout;                    // their staged images
+    std::vector<nli::DBuf<uint8_t>> lmask;
     ~nli_worker()
     {
         if (stream) (void)hipStreamDestroy(stream);
@@ -52,6 +55,11 @@ struct nli_ctx {
 };
 
 namespace {
+
+// batches: 0 = concurrent per-image drivers (the default: faster on the measured workloads,
+// where lockstep groups wait for their slowest image at every scale), 1 = lockstep groups
+// where supported, 2 = concurrent per-image drivers
+std::atomic<int> g_batch_mode{0};
 
 struct ArgError : std::invalid_argument {
     using std::invalid_argument::invalid_argument;
@@ -272,14 +280,66 @@ void run_batch(nli_ctx *ctx, const nli_inpaint_params *params, bool per_image, i
         NLI_HIP(hipStreamCreateWithFlags(&wk->stream, hipStreamNonBlocking));
         ctx->workers.push_back(std::move(wk));
     }
+    // lockstep groups: every image has the same parameters and the lockstep driver
+    // supports them; the images (largest first) are dealt round-robin to `workers`
+    // groups, one per host thread and stream
+    bool lockstep = g_batch_mode.load() == 1;
+    for (int i = 0; i < n && lockstep; i++) {
+        const nli::InpaintConfig &c = cfgs[per_image ? i : 0];
+        lockstep = nli::lockstep_supported(c, W[i], H[i], C) &&
+                   (!per_image || std::memcmp(&c, &cfgs[0], sizeof(c)) == 0);
+    }
+    std::vector<std::vector<int>> groups(workers);
+    for (int k = 0; k < n; k++) groups[k % workers].push_back(order[k]);
     std::atomic<int> next(0);
     std::mutex mu;
     std::string first_error;
     int first_code = 0;
-    auto body = [&](nli_worker *wk) {
+    auto body = [&](nli_worker *wk, int group) {
         try {
             NLI_HIP(hipSetDevice(ctx->device));
-            for (int k = next++; k < n; k = next++) {
+            if (lockstep && !groups[group].empty()) {
+                const std::vector<int> &g = groups[group];
+                const int m = (int)g.size();
+                std::vector<const float *> gin(m);
+                std::vector<const uint8_t *> gmask(m);
+                std::vector<float *> gout(m);
+                std::vector<int> gw(m), gh(m);
+                std::vector<nli::InpaintReport> reps(m);
+                if (staged) {
+                    wk->lin.resize(std::max<size_t>(wk->lin.size(), m));
+                    wk->lout.resize(std::max<size_t>(wk->lout.size(), m));
+                    wk->lmask.resize(std::max<size_t>(wk->lmask.size(), m));
+                }
+                for (int j = 0; j < m; j++) {
+                    const int i = g[j];
+                    const size_t np = (size_t)W[i] * H[i];
+                    gw[j] = W[i];
+                    gh[j] = H[i];
+                    gin[j] = in[i];
+                    gmask[j] = mask[i];
+                    gout[j] = out[i];
+                    if (staged) {
+                        gin[j] = wk->lin[j].ensure(np * C);
+                        gmask[j] = wk->lmask[j].ensure(np);
+                        gout[j] = wk->lout[j].ensure(np * C);
+                        NLI_HIP(hipMemcpyAsync((void *)gin[j], in[i], np * C * sizeof(float), hipMemcpyHostToDevice,
+                                               wk->stream));
+                        NLI_HIP(hipMemcpyAsync((void *)gmask[j], mask[i], np, hipMemcpyHostToDevice, wk->stream));
+                    }
+                }
+                nli::inpaint_lockstep(cfgs[0], m, gin.data(), gmask.data(), gw.data(), gh.data(), C, gout.data(),
+                                      reps.data(), wk->lock, wk->stream);
+                for (int j = 0; j < m; j++) {
+                    const int i = g[j];
+                    if (staged)
+                        NLI_HIP(hipMemcpyAsync(out[i], gout[j], (size_t)W[i] * H[i] * C * sizeof(float),
+                                               hipMemcpyDeviceToHost, wk->stream));
+                    if (stats) report(reps[j], &stats[i]);
+                }
+                NLI_HIP(hipStreamSynchronize(wk->stream));
+            }
+            for (int k = next++; k < n && !lockstep; k = next++) {
                 const int i = order[k];
                 const nli::InpaintConfig &cfg = cfgs[per_image ? i : 0];
                 const size_t np = (size_t)W[i] * H[i];
@@ -318,8 +378,8 @@ void run_batch(nli_ctx *ctx, const nli_inpaint_params *params, bool per_image, i
         next = n;   // stop the other workers after a failure or at the end
     };
     std::vector<std::thread> threads;
-    for (int k = 1; k < workers; k++) threads.emplace_back(body, ctx->workers[k].get());
-    body(ctx->workers[0].get());
+    for (int k = 1; k < workers; k++) threads.emplace_back(body, ctx->workers[k].get(), k);
+    body(ctx->workers[0].get(), 0);
     for (auto &t : threads) t.join();
     if (first_code == NLI_E_EMPTY_SOURCE) throw nli::EmptySource();
     if (first_code == NLI_E_ARGUMENT) throw ArgError(first_error);
@@ -402,6 +462,13 @@ int nli_pm_tiles_per_block(int tiles)
 {
     if (tiles < 0 || tiles > 2) return NLI_E_ARGUMENT;
     nli::set_pm_tiles_per_block(tiles);
+    return NLI_OK;
+}
+
+int nli_batch_mode(int mode)
+{
+    if (mode < 0 || mode > 2) return NLI_E_ARGUMENT;
+    g_batch_mode = mode;
     return NLI_OK;
 }
 

@@ -5,7 +5,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
+#include <memory>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -171,6 +173,46 @@ struct PMConfig {
     int first_step;   // jump-flood step of the first sweep; 0 = automatic
 };
 
+// A table of kernel arguments for batched launches: filled on the host in pinned memory
+// and copied to the device in stream order.  The host side is refilled only after the
+// previous copy from it has completed (an event after each copy), and the device side is
+// overwritten only in stream order, after the kernels that read it.
+struct ArgTable {
+    char *host = nullptr;
+    size_t cap = 0;
+    DBuf<char> dev;
+    hipEvent_t copied = nullptr;
+    ArgTable() = default;
+    ArgTable(const ArgTable &) = delete;
+    ArgTable &operator=(const ArgTable &) = delete;
+    ~ArgTable()
+    {
+        if (host) (void)hipHostFree(host);
+        if (copied) (void)hipEventDestroy(copied);
+    }
+    // host storage for `bytes`, safe to write
+    void *fill(size_t bytes)
+    {
+        if (copied) NLI_HIP(hipEventSynchronize(copied));
+        if (bytes > cap) {
+            if (host) NLI_HIP(hipHostFree(host));
+            host = nullptr;
+            cap = std::max<size_t>(bytes, 4096);
+            NLI_HIP(hipHostMalloc((void **)&host, cap, hipHostMallocDefault));
+        }
+        return host;
+    }
+    // copies the first `bytes` to the device; returns the device address
+    const void *upload(size_t bytes, hipStream_t s)
+    {
+        dev.ensure(bytes);
+        if (!copied) NLI_HIP(hipEventCreateWithFlags(&copied, hipEventDisableTiming));
+        NLI_HIP(hipMemcpyAsync(dev.p, host, bytes, hipMemcpyHostToDevice, s));
+        NLI_HIP(hipEventRecord(copied, s));
+        return dev.p;
+    }
+};
+
 struct PMWork {          // per-call scratch, grown on demand
     DBuf<float> sgrad, tgrad, weights, dist0, dist1;
     DBuf<int32_t> nnf1;
@@ -205,6 +247,32 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
                 int sW, int sH, const float *tgt, const uint8_t *tmask, int tW, int tH, int C,
                 const int32_t *init_nnf, uint64_t seed, int32_t *out_nnf, float *out_dist,
                 PMWork &w, hipStream_t s);
+
+// Several PatchMatch calls in lockstep: one launch per initialisation and per sweep
+// covers the target tiles of every job.  The jobs share the distance kind, channel count
+// and patch side, and pm_batch_supported() holds for each; results equal patchmatch()'s
+// on each job, except that a job whose *active is 0 is skipped by the search kernels.
+struct PMJob {
+    DistDesc d;
+    PMConfig cfg;
+    const float *src;
+    const uint8_t *smask;
+    int sW, sH;
+    const float *tgt;
+    const uint8_t *tmask;
+    int tW, tH, C;
+    const int32_t *init_nnf;
+    uint64_t seed;
+    int32_t *out_nnf;
+    float *out_dist;
+    PMWork *w;
+    const int *active;   // device flag, or null (always run)
+};
+struct PMBatchWork {
+    ArgTable args, starts;
+};
+bool pm_batch_supported(const DistDesc &d, int C, int sW, int sH, const PMConfig &cfg);
+void patchmatch_batch(const PMJob *jobs, int n, PMBatchWork &bw, hipStream_t s);
 
 // Distances of every target to a fixed NNF (FLT_MAX elsewhere).
 void patch_distance_field(const DistDesc &d, const float *src, int sW, int sH, const float *tgt,
@@ -267,6 +335,51 @@ double update(int kind, int patch, float sigma, float lambda, const CGParams &cg
               int n_hole, const uint8_t *ext, const int32_t *nnf, const float *conf, int W, int H,
               UpdWork &w, CGStats *stats, hipStream_t s);
 
+// ---- updates in lockstep (k_update.hip) ----
+// The stopping rule of the inpainting loop (image_inpainting.cpp:389), kept on the device:
+// after the update, *active = total > tol && ++*count < iters.  part: 0 = one update per
+// iteration; 1 and 2 = the first and second of two updates whose totals add (NL-Depth):
+// the first stores its total in *partial, the second adds it and applies the rule.  The
+// new flag also goes to host[0] (pinned, mapped), followed by `stamp` in host[1].
+struct StopRule {
+    int *active = nullptr;   // null: no rule (the update's total is not used)
+    int *count = nullptr;
+    double *total = nullptr;
+    double *partial = nullptr;
+    int *cg_total = nullptr;   // CG iterations summed over channels and updates
+    int iters = 0;
+    float tol = 0.0f;
+    int part = 0;
+    int *host = nullptr;
+    unsigned stamp = 0;
+};
+struct UpdJob {
+    int kind, patch;
+    float sigma, lambda;
+    CGParams cg;
+    float *image;
+    const float *orig;
+    int Cs, c0, nc;
+    const uint8_t *hole;
+    const int *hole_pts;
+    int n_hole;
+    const uint8_t *ext;
+    const int32_t *nnf;
+    const float *conf;
+    int W, H;
+    UpdWork *w;
+    const int *active;   // device flag (0: skip this image), or null
+    StopRule stop;
+};
+struct UpdBatchWork {
+    ArgTable jobs, starts;
+    unsigned poll_seq = 1;
+};
+// Several updates in lockstep: one launch per kernel covers every job.  The jobs share
+// the kind, patch side, channel count and CG precision.  Each job's result equals
+// update()'s.
+void update_batch(const UpdJob *jobs, int n, UpdBatchWork &bw, hipStream_t s);
+
 // ---- the multiscale driver (nli_driver.cpp) ----
 enum InitKind { INIT_BLACK = 0, INIT_AVG = 1, INIT_NONE = 2, INIT_POISSON = 3 };
 
@@ -323,6 +436,40 @@ struct DriverWork {
 
 void inpaint(const InpaintConfig &cfg, const float *d_in, const uint8_t *d_mask, int W, int H, int C,
              float *d_out, InpaintReport &rep, DriverWork &dw, PMWork &pmw, UpdWork &uw, hipStream_t s);
+
+// Several images in lockstep on one stream (same parameters, any sizes): every scale's
+// PatchMatch calls and updates of all images run as single launches (patchmatch_batch,
+// update_batch), and the stopping rule is applied on the device: an image whose
+// iterations have ended is skipped by the kernels while the others continue, and the host
+// learns it one iteration later without waiting.  Each image's output and report equal
+// inpaint()'s.  lockstep_supported() tells whether the parameters and sizes allow it.
+struct LockstepImage {
+    DriverWork dw;
+    PMWork pm;
+    UpdWork uw;
+};
+struct LockstepWork {
+    std::vector<std::unique_ptr<LockstepImage>> img;
+    PMBatchWork pmb;
+    UpdBatchWork ub;
+    DBuf<int> state;       // per image: active, count, CG iterations
+    DBuf<double> totals;   // per image: total, partial
+    int *host = nullptr;   // pinned, mapped: per image [active flag, stamp]
+    int *host_dev = nullptr;
+    size_t host_slots = 0;
+    unsigned stamp = 0;
+    LockstepWork() = default;
+    LockstepWork(const LockstepWork &) = delete;
+    LockstepWork &operator=(const LockstepWork &) = delete;
+    ~LockstepWork()
+    {
+        if (host) (void)hipHostFree(host);
+    }
+};
+bool lockstep_supported(const InpaintConfig &cfg, int W, int H, int C);
+void inpaint_lockstep(const InpaintConfig &cfg, int n, const float *const *d_in, const uint8_t *const *d_mask,
+                      const int *W, const int *H, int C, float *const *d_out, InpaintReport *rep, LockstepWork &lw,
+                      hipStream_t s);
 
 // ---- patch energy, its gradient, extraction/aggregation pair (k_adjoint.hip) ----
 struct EnergyDesc {
