@@ -517,7 +517,7 @@ def assert_psnr_within_0p1db(label, g, r):
 
 # GPU seeds per case: enough that two standard errors of the difference stay within
 # 0.1 dB given the seed-to-seed spreads (reference: 0.25, 0.41, 0.67 dB)
-PROCESS_SEEDS = {"nlmeans256": 40, "nlmedians192": 112, "nlpoisson128": 400, "rgb_nlpoisson128": 160,
+PROCESS_SEEDS = {"nlmeans256": 40, "nlmedians192": 112, "nlpoisson128": 400, "rgb_nlpoisson128": 400,
                  "small256": 48}
 
 
