@@ -194,6 +194,20 @@ int nli_patch_distance(nli_ctx *ctx, const nli_dist_params *dist, const float *s
                        int sH, const float *tgt, int tW, int tH, int C, const int32_t *nnf,
                        const uint8_t *tmask, float *out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * nli_patch_distance on device pointers (src, tgt, nnf, tmask, out), with the same
+ * results, and with no allocation: its scratch (the source and target gradients, the
+ * patch weights) is a static array of NLI_NOALLOC_SCRATCH_BYTES per device, whose
+ * address nli_ctx_create looks up.  Images needing more than that (256 + 2 C (sW sH +
+ * tW tH, the target term only if tgt != src) floats) are refused with NLI_E_ARGUMENT.
+ * All calls on one device share that device's array: calls that overlap on one device
+ * (contexts on different streams) race. */
+#define NLI_NOALLOC_SCRATCH_BYTES (32u << 20)
+int nli_patch_distance_dev_noalloc(nli_ctx *ctx, const nli_dist_params *dist, const float *src, int sW,
+                                   int sH, const float *tgt, int tW, int tH, int C, const int32_t *nnf,
+                                   const uint8_t *tmask, float *out);
+
 /* replaces PatchMatch::calculate (patch_match.h:50-54, patch_match.cpp:57-512).
  * init_nnf = NULL is the reference's empty initial field (random initialisation).
  * out_dist (optional) receives each target's distance (FLT_MAX elsewhere).

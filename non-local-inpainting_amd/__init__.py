@@ -44,7 +44,7 @@ EXPORTS = [
     "nli_patchmatch", "nli_patchmatch_dev", "nli_update", "nli_update_dev", "nli_poisson_fill",
     "nli_inpaint", "nli_inpaint_dev", "nli_inpaint_params_default", "nli_profile_enable",
     "nli_patch_energy", "nli_energy_gradient", "nli_patch_extract", "nli_adjoint_aggregate",
-    "nli_pm_tiles_per_block", "nli_pm_row_scan", "nli_batch_mode",
+    "nli_pm_tiles_per_block", "nli_pm_row_scan", "nli_patch_distance_dev_noalloc", "nli_batch_mode",
     "nli_profile_read", "nli_inpaint_batch", "nli_inpaint_batch_dev", "nli_inpaint_batch_each",
     "nli_inpaint_batch_each_dev", "nli_rgb_to_lab", "nli_lab_to_rgb", "nli_rgb_to_lab_dev",
     "nli_lab_to_rgb_dev", "nli_inpaint_rgb", "nli_inpaint_rgb_dev", "nli_profile_enable_sampled",

@@ -1514,6 +1514,82 @@ void patchmatch_batch(const PMJob *jobs, int n, PMBatchWork &bw, hipStream_t s)
     }
 }
 
+// ---- fixed-field distances without allocation ----
+// Device memory of the allocation-free entry point: the source and target gradients and
+// the patch weights.  Every GPU holds its own copy until the process ends; every call on
+// a GPU uses that GPU's copy, so calls that overlap on one GPU (several streams) race.
+__device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices] = {};   // device address of `scratch` on each device
+}  // namespace
+
+void noalloc_scratch_init()
+{
+    const int dev = current_device();
+    if (dev < 0 || dev >= kMaxDevices) throw std::invalid_argument("device index beyond the scratch table");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return;
+    void *p = nullptr;
+    NLI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<float *>(p);
+}
+
+void patch_distance_field_noalloc(const DistDesc &d, const float *src, int sW, int sH, const float *tgt, int tW,
+                                  int tH, int C, const int32_t *nnf, const uint8_t *tmask, float *out, hipStream_t s)
+{
+    if (d.patch < 1 || d.patch > 15) throw std::invalid_argument("patch side must be in [1, 15]");
+    if (C < 1 || C > 4) throw std::invalid_argument("1 to 4 channels supported");
+    const int dev = current_device();
+    float *base = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        base = (dev >= 0 && dev < kMaxDevices) ? g_scratch_ptr[dev] : nullptr;
+    }
+    if (!base) throw std::logic_error("no-allocation scratch not set up on this device (create a context first)");
+    const bool grad = d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD;
+    const size_t wfl = 256;   // weights region (P^2 <= 225 floats), keeps the gradients 1 KiB aligned
+    const size_t sg = grad ? (size_t)sW * sH * C * 2 : 0, tg = (grad && tgt != src) ? (size_t)tW * tH * C * 2 : 0;
+    if (wfl + sg + tg > kNoAllocScratchFloats)
+        throw std::invalid_argument("image too large for the no-allocation scratch (" +
+                                    std::to_string(kNoAllocScratchFloats * sizeof(float)) +
+                                    " bytes per device): 256 + 2 C (source + distinct target pixels) floats needed");
+    float wh[15 * 15];
+    gaussian_weights(d.patch, d.patch, d.sigma, d.sigma, wh);
+    NLI_HIP(hipMemcpyAsync(base, wh, sizeof(float) * d.patch * d.patch, hipMemcpyHostToDevice, s));
+    NLI_HIP(hipStreamSynchronize(s));   // wh is on the host stack
+    PMArgs a{};
+    a.src = src;
+    a.smask = tmask;   // not read for a fixed field
+    a.sW = sW;
+    a.sH = sH;
+    a.tgt = tgt;
+    a.tmask = tmask;
+    a.tW = tW;
+    a.tH = tH;
+    a.P = d.patch;
+    a.lambda = d.lambda;
+    a.lambda_d = d.lambda_d;
+    a.w = base;
+    if (grad) {
+        a.sgrad = base + wfl;
+        gradient(src, sW, sH, C, base + wfl, s);
+        a.tgrad = a.sgrad;
+        if (tgt != src) {
+            a.tgrad = base + wfl + sg;
+            gradient(tgt, tW, tH, C, base + wfl + sg, s);
+        }
+    }
+    a.dist_out = out;
+    dim3 g((tW + kBlock - 1) / kBlock, tH);
+#define NLI_FIELD(KD, CC, PP) k_dist_field<KD, CC, PP><<<g, kBlock, 0, s>>>(a, (const int2 *)nnf)
+    NLI_DISPATCH(d.kind, C, d.patch, NLI_FIELD);
+#undef NLI_FIELD
+    NLI_LAUNCH_CHECK();
+}
+
 void patch_distance_field(const DistDesc &d, const float *src, int sW, int sH, const float *tgt, int tW, int tH,
                           int C, const int32_t *nnf, const uint8_t *tmask, float *out, PMWork &w, hipStream_t s)
 {

@@ -406,6 +406,13 @@ int nli_ctx_create(int device, nli_ctx **out)
         delete c;
         return NLI_E_DEVICE;
     }
+    try {
+        nli::noalloc_scratch_init();   // the device address of nli_patch_distance_dev_noalloc's scratch
+    } catch (const std::exception &) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return NLI_E_DEVICE;
+    }
     *out = c;
     return NLI_OK;
 }
@@ -795,6 +802,20 @@ int nli_patch_distance(nli_ctx *ctx, const nli_dist_params *dist, const float *s
     });
 }
 
+int nli_patch_distance_dev_noalloc(nli_ctx *ctx, const nli_dist_params *dist, const float *src, int sW, int sH,
+                                   const float *tgt, int tW, int tH, int C, const int32_t *nnf, const uint8_t *tmask,
+                                   float *out)
+{
+    return guard(ctx, [&] {
+        nli::DistDesc d = desc(dist);
+        need(src && tgt && nnf && tmask && out && sW > 0 && sH > 0 && tW > 0 && tH > 0, "bad distance arguments");
+        need(C >= 1 && C <= 4, "1 to 4 channels supported");
+        need(d.kind != NLI_DIST_L2RGBD || C == 4, "L2 RGBD Distance: Channels do not match.");
+        nli::patch_distance_field_noalloc(d, src, sW, sH, tgt, tW, tH, C, nnf, tmask, out, ctx->stream);
+        sync(ctx);
+    });
+}
+
 int nli_patchmatch_dev(nli_ctx *ctx, const nli_dist_params *dist, const nli_pm_params *pm, const float *src,
                        const uint8_t *smask, int sW, int sH, const float *tgt, const uint8_t *tmask, int tW, int tH,
                        int C, const int32_t *init_nnf, uint64_t seed, int32_t *out_nnf, float *out_dist)
@@ -1119,3 +1140,6 @@ int nli_inpaint_rgb_dev(nli_ctx *ctx, const nli_inpaint_params *params, const fl
 }
 
 }  // extern "C"
+
+static_assert(nli::kNoAllocScratchFloats * sizeof(float) == NLI_NOALLOC_SCRATCH_BYTES,
+              "the header's scratch size is the library's");

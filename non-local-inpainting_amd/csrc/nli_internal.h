@@ -248,6 +248,15 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
                 const int32_t *init_nnf, uint64_t seed, int32_t *out_nnf, float *out_dist,
                 PMWork &w, hipStream_t s);
 
+// The same without allocating: scratch in a per-device static array of
+// kNoAllocScratchFloats floats, whose address noalloc_scratch_init() looks up (once per
+// device, with the device current).
+constexpr size_t kNoAllocScratchFloats = (size_t)8 << 20;
+void noalloc_scratch_init();
+void patch_distance_field_noalloc(const DistDesc &d, const float *src, int sW, int sH, const float *tgt, int tW,
+                                  int tH, int C, const int32_t *nnf, const uint8_t *tmask, float *out,
+                                  hipStream_t s);
+
 // Several PatchMatch calls in lockstep: one launch per initialisation and per sweep
 // covers the target tiles of every job.  The jobs share the distance kind, channel count
 // and patch side, and pm_batch_supported() holds for each; results equal patchmatch()'s
