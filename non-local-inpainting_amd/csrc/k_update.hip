@@ -26,6 +26,7 @@
 //   stencil arithmetic, FP64 dot-product accumulation, and the residual recomputed
 //   from an FP64 evaluation of b - A x every `refine_every` iterations.
 #include "nli_internal.h"
+#include "nli_introsort.h"
 
 #include <atomic>
 #include <chrono>
@@ -267,9 +268,14 @@ __global__ __launch_bounds__(kBlock) void k_nlmeans_w(const float *__restrict__ 
     }
 }
 
-// NL-medians (patch_non_local_medians.cpp:23-126): per channel the contributors are
-// ranked by value (equal values in gather order), placed in that order, and the weights
-// subtracted from the total in that order until it drops below half
+// NL-medians (patch_non_local_medians.cpp:23-126): per channel the contributors are put in
+// std::sort's order by value and the weights subtracted from the total in that order until
+// it drops below half.  Without equal values that order is the ranking by value; with
+// every value equal the median is that value whatever the order; otherwise (equal values
+// among different ones) the ranking gives the same median as every other order of equal
+// values unless rounding of the running total could move the crossing to another value
+// (median_order_free), and only then lane 0 of the pixel sorts the (value, weight) pairs
+// in gather order with the restatement of std::sort (nli_introsort.h).
 template <int SLOTS>
 __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict__ img, int Cs, int c0, int nc,
                                                         const int *__restrict__ pts, int n,
@@ -285,6 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
     __shared__ float lv[kNLWaves][kNLPix][kE];
     __shared__ float sv[kNLWaves][kNLPix][kE];
     __shared__ float sw[kNLWaves][kNLPix][kE];
+    __shared__ int sstk[kNLWaves][kNLPix][3 * kIntrosortStack];
     for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
     __syncthreads();
     const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
@@ -310,17 +317,38 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
             int rank[SLOTS];
 #pragma unroll
             for (int s = 0; s < SLOTS; s++) rank[s] = 0;
+            bool tie = false, differ = false;
             for (int f = 0; f < m; f++) {
                 const float u = mv[f];
 #pragma unroll
-                for (int s = 0; s < SLOTS; s++) rank[s] += (u < v[s] || (u == v[s] && f < pos[s])) ? 1 : 0;
+                for (int s = 0; s < SLOTS; s++) {
+                    rank[s] += (u < v[s] || (u == v[s] && f < pos[s])) ? 1 : 0;
+                    if (pos[s] >= 0) {
+                        tie = tie || (u == v[s] && f != pos[s]);
+                        differ = differ || u != v[s];
+                    }
+                }
             }
+            const unsigned gmask = 0xFFFFu;
+            const bool sort = ((__ballot(tie) >> (kNLG * grp)) & gmask) != 0 &&
+                              ((__ballot(differ) >> (kNLG * grp)) & gmask) != 0;
 #pragma unroll
             for (int s = 0; s < SLOTS; s++)
                 if (pos[s] >= 0) {
                     ms[rank[s]] = v[s];
                     mo[rank[s]] = wt[s];
                 }
+            wave_lds_sync();
+            // equal values among different ones: the ranked order (equal values in gather
+            // order) gives the reference's median unless rounding of the running total could
+            // move the crossing to another value; then std::sort's order is formed
+            if (sort && gl == 0 && !median_order_free(ms, mo, m, tw)) {
+                for (int f = 0; f < m; f++) {
+                    ms[f] = mv[f];
+                    mo[f] = mw[f];
+                }
+                introsort_pairs(ms, mo, m, sstk[wave][grp]);
+            }
             wave_lds_sync();
             float total = tw, med = 0.0f;
             for (int e = 0; e < m; e++) {
