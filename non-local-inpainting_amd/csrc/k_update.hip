@@ -274,8 +274,9 @@ __global__ __launch_bounds__(kBlock) void k_nlmeans_w(const float *__restrict__ 
 // every value equal the median is that value whatever the order; otherwise (equal values
 // among different ones) the ranking gives the same median as every other order of equal
 // values unless rounding of the running total could move the crossing to another value
-// (median_order_free), and only then lane 0 of the pixel sorts the (value, weight) pairs
-// in gather order with the restatement of std::sort (nli_introsort.h).
+// (the bound of median_order_free, checked during the walk), and only then lane 0 of the
+// pixel sorts the (value, weight) pairs in gather order with the restatement of std::sort
+// (nli_introsort.h).
 template <int SLOTS>
 __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict__ img, int Cs, int c0, int nc,
                                                         const int *__restrict__ pts, int n,
@@ -317,21 +318,11 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
             int rank[SLOTS];
 #pragma unroll
             for (int s = 0; s < SLOTS; s++) rank[s] = 0;
-            bool tie = false, differ = false;
             for (int f = 0; f < m; f++) {
                 const float u = mv[f];
 #pragma unroll
-                for (int s = 0; s < SLOTS; s++) {
-                    rank[s] += (u < v[s] || (u == v[s] && f < pos[s])) ? 1 : 0;
-                    if (pos[s] >= 0) {
-                        tie = tie || (u == v[s] && f != pos[s]);
-                        differ = differ || u != v[s];
-                    }
-                }
+                for (int s = 0; s < SLOTS; s++) rank[s] += (u < v[s] || (u == v[s] && f < pos[s])) ? 1 : 0;
             }
-            const unsigned gmask = 0xFFFFu;
-            const bool sort = ((__ballot(tie) >> (kNLG * grp)) & gmask) != 0 &&
-                              ((__ballot(differ) >> (kNLG * grp)) & gmask) != 0;
 #pragma unroll
             for (int s = 0; s < SLOTS; s++)
                 if (pos[s] >= 0) {
@@ -339,10 +330,41 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
                     mo[rank[s]] = wt[s];
                 }
             wave_lds_sync();
-            // equal values among different ones: the ranked order (equal values in gather
-            // order) gives the reference's median unless rounding of the running total could
-            // move the crossing to another value; then std::sort's order is formed
-            if (sort && gl == 0 && !median_order_free(ms, mo, m, tw)) {
+            // the walk in ranked order (equal values in gather order), with the exact totals
+            // at the ends of the groups of equal values alongside (median_order_free's bound):
+            // the ranked order gives the reference's median if no values tie up to the
+            // crossing, or if every order of equal values crosses in the same group
+            float total = tw, med = 0.0f;
+            const double unit = (double)tw * (0x1p-24 + 0x1p-50), halfd = (double)half;
+            double S = (double)tw, S_prev = (double)tw;
+            int j_prev = 0, e = 0;
+            bool tie = false, crossed = false;
+            for (; e < m; e++) {
+                if (e > 0 && ms[e] != ms[e - 1]) {
+                    S_prev = S;
+                    j_prev = e;
+                } else if (e > 0) {
+                    tie = true;
+                }
+                total -= mo[e];
+                S -= (double)mo[e];
+                if (total < half) {
+                    med = ms[e];
+                    crossed = true;
+                    break;
+                }
+            }
+            bool ok = !tie;
+            if (crossed) {
+                int j = e + 1;
+                for (; j < m && ms[j] == ms[e]; j++) {
+                    S -= (double)mo[j];
+                    tie = true;
+                }
+                ok = !tie || (tw > 0.0f && S_prev - j_prev * unit >= halfd && S + j * unit < halfd);
+            }
+            // otherwise std::sort's order itself (the same decision in every lane of the pixel)
+            if (!ok && gl == 0) {
                 for (int f = 0; f < m; f++) {
                     ms[f] = mv[f];
                     mo[f] = mw[f];
@@ -350,12 +372,15 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
                 introsort_pairs(ms, mo, m, sstk[wave][grp]);
             }
             wave_lds_sync();
-            float total = tw, med = 0.0f;
-            for (int e = 0; e < m; e++) {
-                total -= mo[e];
-                if (total < half) {
-                    med = ms[e];
-                    break;
+            if (!ok) {
+                total = tw;
+                med = 0.0f;
+                for (int f = 0; f < m; f++) {
+                    total -= mo[f];
+                    if (total < half) {
+                        med = ms[f];
+                        break;
+                    }
                 }
             }
             if (live && gl == 0 && m > 0) newv[(size_t)k * nc + ch] = med;
