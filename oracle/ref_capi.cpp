@@ -4,10 +4,11 @@
  * oracle/_ref/.  TEST INFRASTRUCTURE ONLY: used to pin the oracle against the
  * reference and as the "reference" CPU baseline; never part of the product.
  *
- * The reference's patch_non_local_depth.cpp is not compiled (it links
- * io_utility.cpp, whose iio image I/O needs libpng/jpeg/tiff that are absent),
- * so the NL-Depth row is composed here from the reference's own
- * PatchNonLocalPoisson exactly as patch_non_local_depth.cpp:36-55 does.
+ * The reference's patch_non_local_depth.cpp is not compiled into this library (it
+ * links io_utility.cpp, whose iio image I/O needs libpng/jpeg/tiff that are absent);
+ * the NL-Depth row is composed here from the reference's own PatchNonLocalPoisson as
+ * patch_non_local_depth.cpp:36-55 does, and the real updater is built separately
+ * into libnli_ref_depth.so (ref_depth_capi.cpp).
  */
 #include <algorithm>
 #include <cmath>

@@ -533,3 +533,34 @@ def test_patchmatch_omp_vs_reference_openmp_build(oracle, syn):
     s0 = np.array([O.nnf_energy(oracle, dp, img, oracle.patchmatch(dp, img, src, img, tgt, None, seed=s), tgt)
                    for s in range(1, 13)])
     assert abs(s1.mean() / s0.mean() - 1) <= 0.01
+
+
+@pytest.mark.parametrize("lam,lam_d", [(1.0, 0.0), (0.3, 0.6)])
+def test_depth_update_vs_reference_depth_updater(oracle, syn, lam, lam_d):
+    """The oracle's NL-Depth (patch_non_local_depth.cpp:36-55: NL-Poisson on the RGB
+    channels and on the depth channel, each with tolerance 1e-6 and limit 1000, merged
+    back) against the reference's own PatchNonLocalDepth compiled in place
+    (oracle/_ref/libnli_ref_depth.so, with io_utility.cpp's separate / cat), bit for bit."""
+    import ctypes as ct
+    path = os.path.join(os.path.dirname(O.REF_SO), "libnli_ref_depth.so")
+    if not os.path.exists(path):
+        pytest.skip("reference NL-Depth not built in place (make -C oracle ref)")
+    lib = ct.CDLL(path)
+    lib.ref_depth_update.restype = ct.c_double
+    W, H = 72, 60
+    img = np.concatenate([syn.texture(W, H, 3, 3, 8), syn.depth_planes(W, H, 5, 8)[..., None]], axis=2)
+    img = np.ascontiguousarray(img, np.float32)
+    hole = syn.stripe_holes(W, H, 2, 5, 8)
+    ext, tgt, src, _ = oracle.scale_masks(hole, 7)
+    nnf = oracle.patchmatch(oracle.dist_params("l2rgbd", 7, 10000.0, lam, lam_d), img, src, img, tgt, None, seed=3)
+    conf = oracle.confidence(hole, 5.0, 0.1)
+    start = img + np.float32(0.01)
+    a, ta = oracle.update("nldepth", start, start, hole, ext, nnf, conf, patch=7, lam=lam, lam_d=lam_d)
+    b = np.ascontiguousarray(start.copy())
+    P = ct.c_void_p
+    tb = lib.ref_depth_update(7, ct.c_float(10000.0), ct.c_float(lam), ct.c_float(lam_d),
+                              b.ctypes.data_as(P), start.ctypes.data_as(P), hole.ctypes.data_as(P),
+                              ext.ctypes.data_as(P), np.ascontiguousarray(nnf).ctypes.data_as(P),
+                              conf.ctypes.data_as(P), W, H)
+    assert bitexact(a, b)
+    assert ta == tb
