@@ -694,6 +694,15 @@ public:
     /* inpainting iterations run at each scale of the last process() (index 0 = finest) */
     const std::vector<int> &get_iterations_per_scale() const { return iterations_per_scale_; }
 
+    /* image_inpainting.cpp:303-337: keep the input image pyramid and the NNF of every scale
+     * (the output image pyramid and the mask pyramid are kept anyway) */
+    void keep_intermediate(bool value) { keep_ = value; }
+    std::vector<Image<float>> get_input_image_pyramid() const { return input_pyramid_; }
+    std::vector<Image<float>> get_output_image_pyramid() const { return image_pyramid_; }
+    std::vector<Mask> get_mask_pyramid() const { return mask_pyramid_; }
+    /* the final NNF of each scale, coarsest first (image_inpainting.cpp:409-412) */
+    std::vector<Image<Point>> get_nnf_pyramid() const { return nnf_pyramid_; }
+
     /* image_inpainting.cpp:68-190 */
     Image<float> process(FixedImage<float> in, FixedMask mask)
     {
@@ -722,6 +731,10 @@ public:
                            "downsample(mask)");
             }
         }
+        input_pyramid_.clear();
+        nnf_pyramid_.clear();
+        if (keep_)   // image_inpainting.cpp:109-115: before the coarsest level is initialised
+            for (const Image<float> &im : imgs) input_pyramid_.push_back(im.clone());
         initialize_coarsest(imgs.back(), masks.back());
         iterations_per_scale_.assign((size_t)scales_, 0);
         Image<Point> nnf;
@@ -729,7 +742,10 @@ public:
             Image<float> conf = confidence(masks[i]);
             if (i < scales_ - 1) nnf = propagate_weights(imgs[i], masks[i], conf, imgs[i + 1], masks[i + 1]);
             iterations_per_scale_[(size_t)i] = inpaint_internal(imgs[i], masks[i], conf, nnf);
+            if (keep_) nnf_pyramid_.push_back(nnf);
         }
+        image_pyramid_ = imgs;
+        mask_pyramid_ = masks;
         return imgs.front();
     }
 
@@ -744,6 +760,10 @@ private:
     float rate_, decay_, asym_;
     InitType init_;
     std::vector<int> iterations_per_scale_;
+    bool keep_ = false;
+    std::vector<Image<float>> input_pyramid_, image_pyramid_;
+    std::vector<Mask> mask_pyramid_;
+    std::vector<Image<Point>> nnf_pyramid_;
 
     unsigned patch_side() const { return image_updating_->get_patch_size().size_x; }
 
