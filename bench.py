@@ -13,6 +13,8 @@ device memory before the timed region.  Prints one JSON line.
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
+import contextlib
+import ctypes
 import importlib
 import json
 import os
@@ -77,21 +79,58 @@ def host_cpu():
     return model, os.cpu_count()
 
 
+REF_OMP = os.path.join(ROOT, "oracle", "_ref", "libnli_ref_omp.so")
+
+
+@contextlib.contextmanager
+def c_stdout_to_stderr():
+    """The reference's process() prints its progress with printf: send the C library's
+    stdout to stderr, line-buffered, while it runs, so that stdout keeps the one JSON
+    line."""
+    libc = ctypes.CDLL(None)
+    c_stdout = ctypes.c_void_p.in_dll(libc, "stdout")
+    sys.stdout.flush()
+    libc.fflush(None)
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    libc.setvbuf(c_stdout, None, 1, 0)   # _IOLBF
+    try:
+        yield
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+
+
+def omp_threads(n):
+    """Set the OpenMP thread count of this process (libgomp is shared by the oracle and
+    the reference library)."""
+    ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
+def cpu_params(O, threads, **kw):
+    p = dict(method="nlpoisson", patch=9, sigma=10000.0, lam=0.05, iters=200, tol=0.1, scales=11, coarse=0.1,
+             init="poisson", cg_tol=1e-4, cg_limit=100, seed=1, pm_threads=threads)
+    p.update(kw)
+    return O.InpaintParams(**p)
+
+
 def cpu_baseline(wl, gpu_iterations, threads):
-    """The CPU oracle (the reference's algorithm, OpenMP PatchMatch) on a bounded sample
-    of the same workload: two inpainting iterations per scale, timed per scale; the
-    estimated full-run time multiplies each scale's per-iteration time by the number of
-    iterations the GPU run made at that scale."""
+    """The reference's own CPU path (its sources built in place with OpenMP,
+    oracle/_ref/libnli_ref_omp.so: ImageInpainting::process, OpenMP PatchMatch) on a
+    bounded sample of the same workload: at most two inpainting iterations per scale.
+    The oracle runs the same sample with per-scale timing; the full-run estimate is the
+    oracle's per-scale iteration time x the GPU run's iterations per scale, scaled by the
+    reference/oracle time ratio on the sample.  Without the reference build, the oracle's
+    own estimate is reported (kind "port")."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
+    omp_threads(threads)
     orc = O.Oracle()
-    p = O.InpaintParams(method="nlpoisson", patch=9, sigma=10000.0, lam=0.05, iters=200, tol=0.1, scales=11,
-                        coarse=0.1, init="poisson", cg_tol=1e-4, cg_limit=100, seed=1, pm_threads=threads,
-                        max_iters_per_scale=2)
     t0 = time.perf_counter()
-    _, st = orc.process(wl.image, wl.mask, p)
-    sample_s = time.perf_counter() - t0
+    _, st = orc.process(wl.image, wl.mask, cpu_params(O, threads, max_iters_per_scale=2))
+    port_s = time.perf_counter() - t0
     est = st.seconds_pyramid
     for L in range(11):
         est += st.seconds_propagate[L]
@@ -99,11 +138,53 @@ def cpu_baseline(wl, gpu_iterations, threads):
             est += st.seconds_iterations[L] / st.iterations[L] * gpu_iterations[L]
     H, W = wl.mask.shape
     model, nproc = host_cpu()
-    return dict(value=W * H / 1e6 / est, unit="MPix/s", cores=threads, kind="port", cpu_model=model,
-                host_logical_cpus=nproc,
-                sample=f"oracle process() of the same 1024x1024 workload with 2 inpainting iterations per scale "
-                       f"({sample_s:.1f} s, PatchMatch on {threads} OpenMP threads); estimated full run "
-                       f"{est:.0f} s = per-scale iteration time x the GPU run's iterations per scale")
+    port = dict(value=W * H / 1e6 / est, sample_seconds=port_s, estimated_full_seconds=est)
+    if not os.path.exists(REF_OMP):
+        return dict(value=port["value"], unit="MPix/s", cores=threads, kind="port", cpu_model=model,
+                    host_logical_cpus=nproc,
+                    sample=f"oracle process() of the same 1024x1024 workload with 2 inpainting iterations per "
+                           f"scale ({port_s:.1f} s, PatchMatch on {threads} OpenMP threads); estimated full run "
+                           f"{est:.0f} s = per-scale iteration time x the GPU run's iterations per scale")
+    ref = O.Reference(REF_OMP)
+    with c_stdout_to_stderr():
+        t0 = time.perf_counter()
+        ref.process(wl.image, wl.mask, cpu_params(O, threads, iters=2))
+        ref_s = time.perf_counter() - t0
+    est_ref = est * ref_s / port_s
+    return dict(value=W * H / 1e6 / est_ref, unit="MPix/s", cores=threads, kind="reference", cpu_model=model,
+                host_logical_cpus=nproc, port=port,
+                sample=f"the reference's ImageInpainting::process (its sources built with OpenMP) on the same "
+                       f"1024x1024 workload with at most 2 inpainting iterations per scale ({ref_s:.1f} s on "
+                       f"{threads} OpenMP threads; the oracle took {port_s:.1f} s on the same sample); estimated "
+                       f"full run {est_ref:.0f} s = the oracle's per-scale iteration time x the GPU run's "
+                       f"iterations per scale x {ref_s / port_s:.3f}")
+
+
+def cpu_full(wls, gpu_outs, threads, n):
+    """Full reference runs (ImageInpainting::process to its stopping rule, OpenMP
+    PatchMatch) of the first n images, timed, and their hole PSNR beside the GPU's outputs
+    of the same images."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    omp_threads(threads)
+    ref = O.Reference(REF_OMP)
+    runs = []
+    for k in range(n):
+        wl = wls[k]
+        hole = wl.mask.astype(bool)
+        with c_stdout_to_stderr():
+            t0 = time.perf_counter()
+            out = ref.process(wl.image, wl.mask, cpu_params(O, threads, seed=1 + k))
+            sec = time.perf_counter() - t0
+        p_ref = psnr(out[hole], wl.truth[hole])
+        p_gpu = psnr(gpu_outs[k][hole], wl.truth[hole])
+        runs.append(dict(image=k, seconds=sec, mpix_s=wl.mask.size / 1e6 / sec, psnr_hole_db_reference=p_ref,
+                         psnr_hole_db_gpu=p_gpu, gpu_minus_reference_db=p_gpu - p_ref))
+        log(f"full reference run {k}: {runs[-1]}")
+    d = np.array([r["gpu_minus_reference_db"] for r in runs])
+    return dict(threads=threads, runs=runs, mean_gpu_minus_reference_db=float(d.mean()),
+                standard_error_db=float(d.std(ddof=1) / np.sqrt(len(d))) if len(d) > 1 else None)
 
 
 def bench_rgbd(args, nli, syn):
@@ -231,6 +312,9 @@ def main():
                     help="first run 3 device-to-device copies of MIB MiB (the known-size copy that "
                          "calibrates hardware byte counters)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-full", type=int, default=0, metavar="N",
+                    help="also run the reference's CPU path in full on the first N images (minutes each) and "
+                         "compare hole PSNR with the GPU's outputs")
     ap.add_argument("--fixed-iters", type=int, default=10, metavar="K",
                     help="also time the batch, one image and the plain drop-in with exactly K inpainting "
                          "iterations per scale (stopping tolerance 0), for like-for-like comparisons; 0 = skip")
@@ -294,13 +378,28 @@ def main():
     ms_per_step = elapsed / args.steps * 1e3
     value = len(wls) * W * H / 1e6 / (ms_per_step / 1e3)
 
-    # roofline of the dominant kernel from the event-timed sweeps of the timed region
-    achieved = sweep["work"] / sweep["ms"] / 1e9          # flops per ms -> TFLOP/s
+    # roofline of the dominant kernel.  In the timed steps the event pair of a sampled sweep
+    # brackets that launch on its own stream, but 7 other drivers' kernels share the CUs
+    # meanwhile, so the span is not the kernel's duration.  The kernel's own duration comes
+    # from one image on one driver, every sweep timed (the configuration a kernel trace of
+    # one driver reports); the timed region's figure is kept beside it.
+    ctx.profile(["pm_sweep"], period=1)
+    ctx.inpaint_dev(d_in[0], d_mask[0], d_out[0], W, H, C, params)
+    sw1 = ctx.profile_read("pm_sweep")
+    ctx.profile(False)
+    achieved = sw1["work"] / sw1["ms"] / 1e9            # flops per ms -> TFLOP/s
+    shared = sweep["work"] / sweep["ms"] / 1e9
     roof = dict(kernel="k_pm_sweep (one PatchMatch sweep)", bound="valu", achieved=achieved,
                 peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS, traffic=measured_traffic(),
                 traffic_source="profiles/k_pm_sweep_rgb1024_nlpoisson_traffic.md (bytes per sweep)",
-                launches=sweep["launches"], avg_launch_ms=sweep["ms"] / max(sweep["launches"], 1),
-                sampled=f"one sweep in {args.sweep_sample} (in issue order over all drivers) carries the event pair",
+                launches=sw1["launches"], avg_launch_ms=sw1["ms"] / max(sw1["launches"], 1),
+                run="image 0 of the batch on one driver after the timed steps, same parameters, every sweep "
+                    "event-timed on its stream",
+                timed_region=dict(achieved=shared, frac=shared / FP32_PEAK_TFLOPS, launches=sweep["launches"],
+                                  avg_launch_ms=sweep["ms"] / max(sweep["launches"], 1),
+                                  sampled=f"one sweep in {args.sweep_sample} (in issue order over all drivers) "
+                                          f"carries the event pair; spans include the other drivers' "
+                                          f"concurrent kernels"),
                 note="bound by FP32/FP64 vector (VALU) work, no MFMA: patch distances are not a contraction; "
                      "peak = the FP32 vector peak; algorithmic flops T*(2+R)*P^2*(9C+4) per sweep")
 
@@ -323,18 +422,9 @@ def main():
     compare = None
     if not args.no_compare:
         # the same image alone on one stream (device-resident), and in FP64
-        ctx.profile(["pm_sweep"], period=args.sweep_sample)
         t1 = time.perf_counter()
         st1 = ctx.inpaint_dev(d_in[0], d_mask[0], d_out[0], W, H, C, params)
         single = W * H / 1e6 / (time.perf_counter() - t1)
-        sw1 = ctx.profile_read("pm_sweep")
-        ctx.profile(False)
-        # the dominant kernel alone on the GPU (one driver): the figure a kernel trace of one
-        # image reports; the timed steps' events include the other drivers' concurrent work
-        a1 = sw1["work"] / sw1["ms"] / 1e9
-        roof["isolated"] = dict(achieved=a1, frac=a1 / FP32_PEAK_TFLOPS, launches=sw1["launches"],
-                                avg_launch_ms=sw1["ms"] / max(sw1["launches"], 1),
-                                run="one image on one driver, same parameters, outside the timed steps")
         out1 = ctx.from_device(d_out[0], wls[0].image.shape, np.float32)
         t1 = time.perf_counter()
         ctx.inpaint_dev(d_in[0], d_mask[0], d_out[0], W, H, C, workload_params(nli, nli.CG_FP64))
@@ -415,6 +505,11 @@ def main():
         cpu = cpu_baseline(wls[0], stats[0].iterations, args.cpu_threads)
         log(f"CPU baseline: {cpu}")
 
+    full = None
+    if args.cpu_full > 0:
+        log(f"full reference runs of {args.cpu_full} images on {args.cpu_threads} threads ...")
+        full = cpu_full(wls, outs, args.cpu_threads, min(args.cpu_full, len(wls)))
+
     line = {
         "metric": METRIC, "value": value, "unit": "MPix/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -425,6 +520,7 @@ def main():
                    "cg_precision": args.precision, "workers": args.workers},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "cpu_full_runs": full,
         "psnr_hole_db": float(np.mean(psnrs)),
         "kernels": kern,
         "compare": compare,

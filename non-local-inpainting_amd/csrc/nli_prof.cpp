@@ -19,7 +19,7 @@ struct Rec {
 
 struct State {
     std::mutex mu;
-    unsigned mask = 0;   // bit k: kernel k is timed
+    std::atomic<unsigned> mask{0};       // bit k: kernel k is timed (read by every launching thread)
     std::atomic<unsigned> period{1};     // time one launch in `period` of each kernel
     std::atomic<unsigned long long> seen[TK_COUNT] = {};
     std::vector<Rec> open;           // begun, not ended (per kernel at most one)
@@ -86,7 +86,7 @@ thread_local bool t_sampled[TK_COUNT] = {};
 void begin(int kernel, hipStream_t stream)
 {
     State &s = st();
-    if (!((s.mask >> kernel) & 1u)) return;
+    if (!((s.mask.load(std::memory_order_relaxed) >> kernel) & 1u)) return;
     t_sampled[kernel] = s.seen[kernel].fetch_add(1, std::memory_order_relaxed) % s.period.load() == 0;
     if (!t_sampled[kernel]) return;
     std::lock_guard<std::mutex> g(s.mu);
@@ -98,7 +98,7 @@ void begin(int kernel, hipStream_t stream)
 void end(int kernel, hipStream_t stream, double work, double units)
 {
     State &s = st();
-    if (!((s.mask >> kernel) & 1u) || !t_sampled[kernel]) return;
+    if (!((s.mask.load(std::memory_order_relaxed) >> kernel) & 1u) || !t_sampled[kernel]) return;
     t_sampled[kernel] = false;
     std::lock_guard<std::mutex> g(s.mu);
     for (size_t i = 0; i < s.open.size(); i++)
