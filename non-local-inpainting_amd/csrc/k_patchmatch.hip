@@ -608,13 +608,14 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
     }
 }
 
-// ---- sweep with per-wave candidate pools (gradient distances, compile-time patch) ----
-// A block takes two target tiles; wave w takes row w of both (32 targets).  The sweep's
-// candidates -- the 4 jump-flood neighbours' shifted matches, then, around the
+// ---- sweep with per-wave candidate pools (compile-time patch) ----
+// A block takes one or two target tiles; wave w takes row w of them (16 or 32 targets).
+// The sweep's candidates -- the 4 jump-flood neighbours' shifted matches, then, around the
 // resulting best, the reference's random search (windows max(W,H)/2^k, at most `shots`
 // tries, the first in-source draw kept) -- are first written to LDS per wave and then
 // evaluated as a pool: a lane whose candidate is pruned by early termination takes the
-// next one, so lanes do not wait for the slowest candidate of a round.  A candidate equal
+// next one, so lanes do not wait for the slowest candidate of a round.  The two phases
+// share one pool (see sweep_pool_body).  A candidate equal
 // to the current best is dropped: its distance is exactly the current one and the
 // reference's strict comparison never takes it.  Each target's best (distance, order) is
 // an LDS 64-bit atomic minimum; a candidate is pruned only when its partial sum exceeds
@@ -653,19 +654,19 @@ constexpr uint64_t kNoKey = ((uint64_t)0x7F800000u << 32) | 0xFFFFFFFFu;   // (+
 // the target tiles are kept pixel-interleaved: F features padded to a multiple of 4
 // floats per pixel, so one pixel's features are 16-byte LDS reads at immediate offsets
 __host__ __device__ constexpr int pool_feat_stride(int F) { return (F + 3) & ~3; }
-__host__ __device__ inline int pool_tasks_per_target(int R) { return R > 4 ? R : 4; }
+__host__ __device__ inline int pool_tasks_per_target(int R) { return 4 + R; }   // orders 0-3: propagation, 4 + wi: window wi
 __host__ __device__ inline size_t pool_tables_offset(int F, int P, int tpb)
 {
     const int r = P / 2;
     const size_t tiles = (size_t)tpb * pool_feat_stride(F) * (TX + 2 * r) * (TY + 2 * r);
     return ((tiles + (size_t)P * P) * sizeof(float) + 15) & ~(size_t)15;
 }
-// per wave: keys (u64), best (int2), distance (float) per target; candidate (int) and
-// list entry (u16) per task
+// per wave: two keys (u64), best (int2), distance (float) per target; candidate (int)
+// and list entry (u16) per task
 __host__ __device__ inline size_t pool_wave_bytes(int R, int tpb)
 {
     const size_t targets = (size_t)tpb * TX;
-    return targets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)) +
+    return targets * (2 * sizeof(uint64_t) + sizeof(int2) + sizeof(float)) +
            targets * pool_tasks_per_target(R) * (sizeof(int) + sizeof(uint16_t));
 }
 template <int FP>
@@ -676,15 +677,21 @@ struct TgtLdsI {
 };
 
 struct WavePool {
-    uint64_t *keys;   // [kWaveTargets] best (distance, order) of the running pool
+    uint64_t *keys;   // [2 kWaveTargets] best (distance, order) of the running pool: tasks of
+                      // order < 4 (propagation, row scan) in [t], random search in [kWaveTargets + t]
     int2 *best;       // [kWaveTargets] current best match
     float *bd;        // [kWaveTargets] its distance (-1: not a target)
-    uint32_t *cand;   // [kWaveTargets * tasks]: pack_xy(cx, cy) for task order * 32 + target
+    uint32_t *cand;   // [kWaveTargets * tasks]: pack_xy(cx, cy) for task order * kWaveTargets + target
     uint16_t *list;   // [kWaveTargets * tasks]: drawn tasks
 };
 
-// Evaluates the `nvalid` listed tasks of a wave (task j: target j % 32, order j / 32,
-// candidate cand[j]); records each completed candidate in keys[target] by atomic minimum.
+// Evaluates the `nvalid` listed tasks of a wave (task j: target j % kWaveTargets, order
+// j / kWaveTargets, candidate cand[j]); records each completed candidate by atomic minimum
+// in keys[target] (order < 4) or keys[kWaveTargets + target] (order >= 4), and prunes it
+// against that key and the target's current distance.  A task of order >= 4 (random
+// search) is dropped as soon as keys[target] lies below the target's distance: then
+// propagation has improved the target and its random search runs again around the new
+// best (sweep_pool_body), so these results would be discarded.
 template <int KIND, int C, int P, int FP, bool LAST_FIRST, int TPB>
 __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const float *tiles, int LW, int tile_floats,
                          int wave, const WavePool &wp, int nvalid)
@@ -697,8 +704,8 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
     const double lamd = (double)a.lambda_d, lamd1 = (double)(1 - a.lambda_d);
     const size_t stride = (size_t)a.sW * C;
     bool active = false;
-    int t = 0, order = 0, cy = 0, dy = 0, last = P;
-    float bound = 0.0f;
+    int t = 0, order = 0, kslot = 0, cy = 0, dy = 0, last = P;
+    float bound = 0.0f, bd_t = 0.0f;
     const float *row = a.src;
     double dist = 0.0, gdist = 0.0, dist_d = 0.0, gdist_d = 0.0;   // gradient distances (FP64)
     float facc = 0.0f;                                             // L2 / L1 norms (FP32)
@@ -709,7 +716,7 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
         return v;
     };
     auto record = [&](float d) {
-        if (d <= bound) atomicMin((unsigned long long *)&wp.keys[t], (unsigned long long)pool_key(d, order));
+        if (d <= bound) atomicMin((unsigned long long *)&wp.keys[kslot], (unsigned long long)pool_key(d, order));
     };
     int next = 0;
     for (;;) {
@@ -721,10 +728,12 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
                 const int j = wp.list[LAST_FIRST ? nvalid - 1 - k : k];
                 t = j % kWaveTargets;
                 order = j / kWaveTargets;
+                kslot = order < 4 ? t : kWaveTargets + t;
                 const int2 cxy = unpack_xy(wp.cand[j]);
                 const int cx = cxy.x;
                 cy = cxy.y;
-                bound = fminf(wp.bd[t], key_dist(wp.keys[t]));
+                bd_t = wp.bd[t];
+                bound = fminf(bd_t, key_dist(wp.keys[kslot]));
                 dy = 0;
                 row = a.src + ((size_t)(cy - r) * a.sW + (cx - r)) * C;
                 if constexpr (GRAD) {
@@ -740,7 +749,8 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
         next += __popcll(m);
         if (__ballot(active) == 0) break;
         if (active) {
-            bound = fminf(bound, key_dist(wp.keys[t]));
+            bound = fminf(bound, key_dist(wp.keys[kslot]));
+            if (kslot != t && key_dist(wp.keys[t]) < bd_t) bound = -1.0f;   // superseded: the first check drops it
             const TgtLdsI<FP> tg{tiles + (t / TX) * tile_floats, LW, t % TX, wave};
             if constexpr (GRAD) {
                 // one patch row per pass: row dy is in A, row dy + 1 goes to B, then B -> A
@@ -816,9 +826,9 @@ __device__ void sweep_pool_body(const PMArgs &a, int slot)
     {
         char *pb = reinterpret_cast<char *>(plds) + pool_tables_offset(F, P, TPB) + wave * pool_wave_bytes(R, TPB);
         wp.keys = reinterpret_cast<uint64_t *>(pb);
-        wp.best = reinterpret_cast<int2 *>(pb + kWaveTargets * sizeof(uint64_t));
-        wp.bd = reinterpret_cast<float *>(pb + kWaveTargets * (sizeof(uint64_t) + sizeof(int2)));
-        wp.cand = reinterpret_cast<uint32_t *>(pb + kWaveTargets * (sizeof(uint64_t) + sizeof(int2) + sizeof(float)));
+        wp.best = reinterpret_cast<int2 *>(pb + 2 * kWaveTargets * sizeof(uint64_t));
+        wp.bd = reinterpret_cast<float *>(pb + kWaveTargets * (2 * sizeof(uint64_t) + sizeof(int2)));
+        wp.cand = reinterpret_cast<uint32_t *>(pb + kWaveTargets * (2 * sizeof(uint64_t) + sizeof(int2) + sizeof(float)));
         wp.list = reinterpret_cast<uint16_t *>(wp.cand + kWaveTargets * ntask);
     }
 
@@ -829,6 +839,38 @@ __device__ void sweep_pool_body(const PMArgs &a, int slot)
         const int tile = s < a.ntiles ? a.tiles[s] : -1;
         tx0[ti] = tile >= 0 ? (tile % a.ntiles_x) * TX : -1;
         ty0[ti] = tile >= 0 ? (tile / a.ntiles_x) * TY : -1;
+    }
+    // targets of this wave: t = 16 * tile + column, row `wave` of the tile
+    // (returns whether target t's tile exists)
+    auto target_coords = [&](int t, int &x, int &y) {
+        int bx = tx0[0], by = ty0[0];   // selected, not indexed: the arrays stay in registers
+#pragma unroll
+        for (int ti = 1; ti < kTilesPerBlock; ti++)
+            if (t / TX == ti) {
+                bx = tx0[ti];
+                by = ty0[ti];
+            }
+        x = bx + t % TX;
+        y = by + wave;
+        return bx >= 0;
+    };
+    // Lanes below kWaveTargets load their target's mask, match and distance here, so the
+    // loads travel with the tile's; a target is valid exactly when its distance in the
+    // wave's table is >= 0 (-1 otherwise), which the candidate generators below rely on.
+    bool my_valid = false;
+    int2 my_best = make_int2(-1, -1);
+    float my_bd = -1.0f;
+    if (lw < kWaveTargets) {
+        int x, y;
+        const bool tile = target_coords(lw, x, y);
+        const bool inb = tile && x >= r && y >= r && x < a.tW - r && y < a.tH - r;
+        const size_t p = inb ? (size_t)a.tW * y + x : 0;
+        const uint8_t m = a.tmask[p];
+        const int2 n = a.nnf_in[p];
+        const float d = a.dist_in[p];
+        my_valid = inb && m;
+        my_best = my_valid ? n : make_int2(-1, -1);
+        my_bd = my_valid ? d : -1.0f;
     }
     for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = a.w[i];
     for (int i = threadIdx.x; i < kTilesPerBlock * plane; i += blockDim.x) {
@@ -848,36 +890,113 @@ __device__ void sweep_pool_body(const PMArgs &a, int slot)
     }
     __syncthreads();
 
-    // targets of this wave: t = 16 * tile + column, row `wave` of the tile
-    auto target_xy = [&](int t, int &x, int &y) {
-        const int ti = t / TX;
-        x = tx0[ti] + t % TX;
-        y = ty0[ti] + wave;
-        return tx0[ti] >= 0 && x < a.tW && y < a.tH && is_target(a, x, y);
-    };
     if (lw < kWaveTargets) {
-        int x, y;
-        const bool valid = target_xy(lw, x, y);
-        const size_t p = valid ? (size_t)a.tW * y + x : 0;
-        wp.best[lw] = valid ? a.nnf_in[p] : make_int2(-1, -1);
-        wp.bd[lw] = valid ? a.dist_in[p] : -1.0f;
+        wp.best[lw] = my_best;
+        wp.bd[lw] = my_bd;
         wp.keys[lw] = kNoKey;
+        wp.keys[kWaveTargets + lw] = kNoKey;
     }
     wave_sync();
 
-    // phase 1: propagation candidates, task order k * 32 + t (k: left, right, up, down)
-    int nvalid = 0;
+    // Random-search candidates of the targets in `which` (bit t: target t), around each
+    // target's current best: the reference's windows max(W,H) / 2^wi, at most `shots` draws
+    // per window, the first in-source draw kept unless it is the centre.  Task j = (4 + wi)
+    // * kWaveTargets + t.  A lane takes the (wi, t) pairs lw + 64 q; the first draws of all
+    // its pairs are formed and their source-mask bytes loaded together, and only draws that
+    // miss the source continue the pair's sequence (regenerated from its key): the same
+    // candidates as drawing each pair to its end in turn.  Appends to the list from nv.
+    auto gen_random = [&](uint64_t which, int nv) {
+        constexpr int kQ = (kMaxR * kWaveTargets + 63) / 64;
+        const int npairs = R * kWaveTargets;
+        uint32_t first[kQ];
+        uint32_t first_in = 0;   // bit q: the first draw of pair q lies on a source-mask pixel
+#pragma unroll
+        for (int q = 0; q < kQ; q++) {
+            const int jp = q * 64 + lw;
+            first[q] = 0xFFFFFFFFu;
+            if (jp < npairs && a.shots > 0) {
+                const int t = jp % kWaveTargets, wi = jp / kWaveTargets;
+                if (((which >> t) & 1ull) && wp.bd[t] >= 0.0f) {
+                    int x, y;
+                    target_coords(t, x, y);
+                    const int2 ctr = wp.best[t];
+                    Rng rng = make_rng(a.seed, a.stream * 32u + (uint32_t)wi, (uint32_t)((size_t)a.tW * y + x));
+                    const int ws = a.maxw >> wi;
+                    const int x_min = max(ctr.x - ws, 0), y_min = max(ctr.y - ws, 0);
+                    const int x_max = min(ctr.x + ws + 1, a.sW), y_max = min(ctr.y + ws + 1, a.sH);
+                    const int cx = x_min + rng.below(x_max - x_min);
+                    const int cy = y_min + rng.below(y_max - y_min);
+                    first[q] = pack_xy(cx, cy);
+                    if ((unsigned)cx < (unsigned)a.sW && (unsigned)cy < (unsigned)a.sH &&
+                        a.smask[(size_t)a.sW * cy + cx])
+                        first_in |= 1u << q;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kQ; q++) {
+            const int jp = q * 64 + lw;
+            bool ok = false;
+            if (first[q] != 0xFFFFFFFFu) {
+                const int t = jp % kWaveTargets, wi = jp / kWaveTargets;
+                const int2 ctr = wp.best[t];
+                const int2 c0 = unpack_xy(first[q]);
+                int cx = c0.x, cy = c0.y;
+                bool found = cx >= r && cy >= r && cx < a.sW - r && cy < a.sH - r && ((first_in >> q) & 1u);
+                if (!found) {
+                    int x, y;
+                    target_coords(t, x, y);
+                    Rng rng = make_rng(a.seed, a.stream * 32u + (uint32_t)wi, (uint32_t)((size_t)a.tW * y + x));
+                    const int ws = a.maxw >> wi;
+                    const int x_min = max(ctr.x - ws, 0), y_min = max(ctr.y - ws, 0);
+                    const int x_max = min(ctr.x + ws + 1, a.sW), y_max = min(ctr.y + ws + 1, a.sH);
+                    rng.below(x_max - x_min);   // the first draw, already tried
+                    rng.below(y_max - y_min);
+                    for (int k = 1; k < a.shots && !found; k++) {
+                        cx = x_min + rng.below(x_max - x_min);
+                        cy = y_min + rng.below(y_max - y_min);
+                        found = is_source(a, cx, cy);
+                    }
+                }
+                if (found) {
+                    ok = cx != ctr.x || cy != ctr.y;
+                    if (ok) wp.cand[4 * kWaveTargets + jp] = pack_xy(cx, cy);
+                }
+            }
+            const uint64_t m = __ballot(ok);   // pairs jp >= npairs are never ok
+            if (ok) wp.list[nv + lane_rank(m)] = (uint16_t)(4 * kWaveTargets + jp);
+            nv += __popcll(m);
+        }
+        return nv;
+    };
+
+    // Phases 1 and 2 in one pool.  Phase 1 (propagation): the 4 neighbours at the
+    // jump-flood step propose their shifted matches, task order k (left, right, up, down).
+    // Phase 2 (random search) belongs around the phase-1 best; its candidates are drawn
+    // around the current best and evaluated in the same pool, with their own running
+    // minimum (keys[kWaveTargets + t]).  Where propagation improves a target, those draws
+    // are discarded and the target's random search runs again around its new best.  So
+    // every target ends with exactly the candidates and choices of the two phases in turn,
+    // while the pool rarely waits for the propagation tasks alone: in the inpainting loop
+    // propagation seldom improves a target.
+    int nvalid = gen_random(~0ull, 0);
     for (int base = 0; base < 4 * kWaveTargets; base += 64) {
         const int j = base + lw;
         const int t = j % kWaveTargets, k = j / kWaveTargets;
         bool ok = false;
-        int x, y;
-        if (wp.bd[t] >= 0.0f && target_xy(t, x, y)) {
+        if (wp.bd[t] >= 0.0f) {
+            int x, y;
+            target_coords(t, x, y);
             const int st = a.step;
             const int ox = k == 0 ? -st : (k == 1 ? st : 0);
             const int oy = k == 2 ? -st : (k == 3 ? st : 0);
-            if (is_target(a, x + ox, y + oy)) {
-                int2 e = a.nnf_in[(size_t)a.tW * (y + oy) + x + ox];
+            const int nx = x + ox, ny = y + oy;
+            // the neighbour's mask and match are loaded together (is_target's test)
+            const bool inb = nx >= r && ny >= r && nx < a.tW - r && ny < a.tH - r;
+            const size_t q = inb ? (size_t)a.tW * ny + nx : 0;
+            const uint8_t nm = a.tmask[q];
+            int2 e = a.nnf_in[q];
+            if (inb && nm) {
                 e.x -= ox;
                 e.y -= oy;
                 const int2 cur = wp.best[t];
@@ -892,59 +1011,42 @@ __device__ void sweep_pool_body(const PMArgs &a, int slot)
         nvalid += __popcll(m);
     }
     wave_sync();
-    run_pool<KIND, C, P, FP, false, TPB>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);
+    // the list is evaluated from its end: propagation first, then the windows smallest first
+    run_pool<KIND, C, P, FP, true, TPB>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);
     wave_sync();
+    bool again = false;
     if (lw < kWaveTargets) {
-        const uint64_t k = wp.keys[lw];
-        if (wp.bd[lw] >= 0.0f && key_dist(k) < wp.bd[lw]) {
-            wp.best[lw] = unpack_xy(wp.cand[(int)(uint32_t)k * kWaveTargets + lw]);
-            wp.bd[lw] = key_dist(k);
-        }
-        wp.keys[lw] = kNoKey;
-    }
-    wave_sync();
-
-    // phase 2: random search around the phase-1 best, task order wi * 32 + t
-    nvalid = 0;
-    for (int base = 0; base < R * kWaveTargets; base += 64) {
-        const int j = base + lw;
-        bool ok = false;
-        if (j < R * kWaveTargets) {
-            const int t = j % kWaveTargets, wi = j / kWaveTargets;
-            int x, y;
-            if (wp.bd[t] >= 0.0f && target_xy(t, x, y)) {
-                const int2 ctr = wp.best[t];
-                Rng rng = make_rng(a.seed, a.stream * 32u + (uint32_t)wi, (uint32_t)((size_t)a.tW * y + x));
-                const int ws = a.maxw >> wi;
-                const int x_min = max(ctr.x - ws, 0), y_min = max(ctr.y - ws, 0);
-                const int x_max = min(ctr.x + ws + 1, a.sW), y_max = min(ctr.y + ws + 1, a.sH);
-                for (int k = 0; k < a.shots; k++) {
-                    const int cx = x_min + rng.below(x_max - x_min);
-                    const int cy = y_min + rng.below(y_max - y_min);
-                    if (is_source(a, cx, cy)) {
-                        ok = cx != ctr.x || cy != ctr.y;
-                        if (ok) wp.cand[j] = pack_xy(cx, cy);
-                        break;
-                    }
-                }
+        const uint64_t kp = wp.keys[lw], kr = wp.keys[kWaveTargets + lw];
+        if (wp.bd[lw] >= 0.0f) {
+            if (key_dist(kp) < wp.bd[lw]) {   // propagation improved: search again around it
+                wp.best[lw] = unpack_xy(wp.cand[(int)(uint32_t)kp * kWaveTargets + lw]);
+                wp.bd[lw] = key_dist(kp);
+                again = true;
+            } else if (key_dist(kr) < wp.bd[lw]) {
+                wp.best[lw] = unpack_xy(wp.cand[(int)(uint32_t)kr * kWaveTargets + lw]);
+                wp.bd[lw] = key_dist(kr);
             }
         }
-        const uint64_t m = __ballot(ok);
-        if (ok) wp.list[nvalid + lane_rank(m)] = (uint16_t)j;
-        nvalid += __popcll(m);
-    }
-    wave_sync();
-    run_pool<KIND, C, P, FP, true, TPB>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);   // smallest windows first
-    wave_sync();
-    if (lw < kWaveTargets) {
-        const uint64_t k = wp.keys[lw];
-        if (wp.bd[lw] >= 0.0f && key_dist(k) < wp.bd[lw]) {
-            wp.best[lw] = unpack_xy(wp.cand[(int)(uint32_t)k * kWaveTargets + lw]);
-            wp.bd[lw] = key_dist(k);
-        }
         wp.keys[lw] = kNoKey;
+        wp.keys[kWaveTargets + lw] = kNoKey;
     }
+    const uint64_t redo = __ballot(again);
     wave_sync();
+    if (redo) {
+        nvalid = gen_random(redo, 0);
+        wave_sync();
+        run_pool<KIND, C, P, FP, true, TPB>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);
+        wave_sync();
+        if (lw < kWaveTargets) {
+            const uint64_t kr = wp.keys[kWaveTargets + lw];
+            if (wp.bd[lw] >= 0.0f && key_dist(kr) < wp.bd[lw]) {
+                wp.best[lw] = unpack_xy(wp.cand[(int)(uint32_t)kr * kWaveTargets + lw]);
+                wp.bd[lw] = key_dist(kr);
+            }
+            wp.keys[kWaveTargets + lw] = kNoKey;
+        }
+        wave_sync();
+    }
 
     // phase 3 (row scan): the reference's sequential scan carries an improvement along a
     // whole row within one sweep (patch_match.cpp:197-232: each target first tries the
@@ -990,13 +1092,12 @@ __device__ void sweep_pool_body(const PMArgs &a, int slot)
         }
     }
 
-    if (lw < kWaveTargets) {
+    if (lw < kWaveTargets && my_valid) {
         int x, y;
-        if (target_xy(lw, x, y)) {
-            const size_t p = (size_t)a.tW * y + x;
-            a.nnf_out[p] = wp.best[lw];
-            a.dist_out[p] = wp.bd[lw];
-        }
+        target_coords(lw, x, y);
+        const size_t p = (size_t)a.tW * y + x;
+        a.nnf_out[p] = wp.best[lw];
+        a.dist_out[p] = wp.bd[lw];
     }
 }
 
@@ -1245,24 +1346,25 @@ int tuned_tiles_per_block(uint64_t key)
 template <class Launch>
 int tune_tiles_per_block(uint64_t key, Launch &&launch, hipStream_t s)
 {
-    hipEvent_t ev[kMaxTilesPerBlock + 1];
+    // both arrangements twice, in the order 1 2 2 1, so that neither gains from running
+    // after the other (caches, clocks); the smaller total wins
+    static_assert(kMaxTilesPerBlock == 2, "the tuning order assumes two arrangements");
+    constexpr int order[4] = {1, 2, 2, 1};
+    hipEvent_t ev[5];
     for (hipEvent_t &e : ev) NLI_HIP(hipEventCreate(&e));
     NLI_HIP(hipEventRecord(ev[0], s));
-    for (int t = 1; t <= kMaxTilesPerBlock; t++) {
-        launch(t);
-        NLI_HIP(hipEventRecord(ev[t], s));
+    for (int i = 0; i < 4; i++) {
+        launch(order[i]);
+        NLI_HIP(hipEventRecord(ev[i + 1], s));
     }
-    NLI_HIP(hipEventSynchronize(ev[kMaxTilesPerBlock]));
-    int best = 1;
-    float best_ms = 0.0f;
-    for (int t = 1; t <= kMaxTilesPerBlock; t++) {
+    NLI_HIP(hipEventSynchronize(ev[4]));
+    float total[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < 4; i++) {
         float ms = 0.0f;
-        NLI_HIP(hipEventElapsedTime(&ms, ev[t - 1], ev[t]));
-        if (t == 1 || ms < best_ms) {
-            best = t;
-            best_ms = ms;
-        }
+        NLI_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        total[order[i]] += ms;
     }
+    const int best = total[2] < total[1] ? 2 : 1;
     for (hipEvent_t e : ev) NLI_HIP(hipEventDestroy(e));
     std::lock_guard<std::mutex> g(g_tune_mu);
     return g_tuned.emplace(std::make_pair(current_device(), key), best).first->second;
