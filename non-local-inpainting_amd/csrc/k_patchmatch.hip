@@ -393,7 +393,7 @@ __global__ void k_tile_flags(const uint8_t *tmask, int W, int H, int ntx, int nt
 // One thread per target of a 16x4 tile; the tile's target features (with the patch
 // halo) are staged in LDS first, as in the sweeps.
 template <int KIND, int C, int PS>
-__device__ void pm_init_body(const PMArgs &a, int tile_slot)
+__device__ __attribute__((always_inline)) inline void pm_init_body(const PMArgs &a, int tile_slot)
 {
     extern __shared__ float init_lds[];
     constexpr int F = Feat<KIND>::grad ? 3 * C : C;
@@ -808,7 +808,7 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
 }
 
 template <int KIND, int C, int PS, int TPB>
-__device__ void sweep_pool_body(const PMArgs &a, int slot)
+__device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMArgs &a, int slot)
 {
     constexpr int kTilesPerBlock = TPB;
     constexpr int kWaveTargets = TPB * TX;
@@ -1134,10 +1134,12 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool_batch(const PMArgs *__
     const int img = batch_image(start, nimg, slot);
     PMArgs a = args[img];
     if (a.active && *a.active == 0) return;
-    a.nnf_in = a.nnf_pp[k & 1];
-    a.dist_in = a.dist_pp[k & 1];
-    a.nnf_out = a.nnf_pp[(k + 1) & 1];
-    a.dist_out = a.dist_pp[(k + 1) & 1];
+    // selected, not indexed by k: an indexed member would put the whole copy in scratch
+    const bool odd = (k & 1) != 0;
+    a.nnf_in = odd ? a.nnf_pp[1] : a.nnf_pp[0];
+    a.dist_in = odd ? a.dist_pp[1] : a.dist_pp[0];
+    a.nnf_out = odd ? a.nnf_pp[0] : a.nnf_pp[1];
+    a.dist_out = odd ? a.dist_pp[0] : a.dist_pp[1];
     a.step = a.s0 >> (k % (a.levels + 1));
     a.stream = (uint32_t)k;
     sweep_pool_body<KIND, C, PS, TPB>(a, slot - start[img]);

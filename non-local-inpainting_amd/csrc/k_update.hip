@@ -184,7 +184,7 @@ __device__ inline void wave_lds_sync()
 // position (-1: not a contributor); returns their count m and writes the weights to
 // lw[pos].  Every lane of the wavefront calls it.
 template <int SLOTS>
-__device__ inline int nl_gather(bool live, int p, int W, int H, int P, const uint8_t *__restrict__ hole,
+__device__ __attribute__((always_inline)) inline int nl_gather(bool live, int p, int W, int H, int P, const uint8_t *__restrict__ hole,
                                 const int2 *__restrict__ nnf, const float *__restrict__ conf, const float *wl,
                                 float *lw, int (&src)[SLOTS], float (&wt)[SLOTS], int (&pos)[SLOTS])
 {
@@ -419,8 +419,11 @@ __global__ void k_write_back(float *__restrict__ img, int Cs, int c0, int nc, co
 // extended-domain flags, NNF entries and confidences, then every valid centre's source
 // samples (colour, right and lower neighbour); then the centres are accumulated in
 // row-major order.  A row costs two dependent memory trips instead of three per centre.
+// The kernel bodies below are shared by the single-image and the batched kernels; with
+// two callers the compiler would otherwise keep the large ones as real calls (248 VGPRs
+// and scratch for the coefficient kernels instead of 117-197 VGPRs inlined).
 template <int NC, int CH>
-__device__ inline void poisson_coef_rows(int qx, int qy, const float *__restrict__ img, int Cs, int c0,
+__device__ __attribute__((always_inline)) inline void poisson_coef_rows(int qx, int qy, const float *__restrict__ img, int Cs, int c0,
                                          const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
                                          const float *__restrict__ conf, const float *__restrict__ wl, int P,
                                          float lambda, int W, int H, double *__restrict__ a1,
@@ -501,7 +504,7 @@ __device__ inline void poisson_coef_rows(int qx, int qy, const float *__restrict
 // neighbour that is itself a hole point is left to its own thread; one that is the
 // neighbour of two hole points is written twice with the same values.
 template <int NC, int CH>
-__device__ inline void poisson_coef_body(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
+__device__ __attribute__((always_inline)) inline void poisson_coef_body(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
                                          const float *__restrict__ img, int Cs, int c0,
                                          const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
                                          const float *__restrict__ conf, const float *__restrict__ w, int P,
@@ -547,7 +550,7 @@ __global__ __launch_bounds__(kBlock) void k_poisson_coef(const int *__restrict__
 // of two per patch row.  Grid row 0: hole points, 1 and 2: their left and top
 // neighbours that are not hole points.
 template <int NC, int SLOTS>
-__device__ inline void poisson_coef16_body(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
+__device__ __attribute__((always_inline)) inline void poisson_coef16_body(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
                                            const float *__restrict__ img, int Cs, int c0,
                                            const uint8_t *__restrict__ ext, const int2 *__restrict__ nnf,
                                            const float *__restrict__ conf, const float *__restrict__ w, int P,
@@ -732,7 +735,7 @@ __device__ inline double lap(const Stencil &s, const int4 &nb, const double *v, 
 // stencil and neighbour indices (reference coefficient choice: top/left use a2 of that
 // neighbour, bottom/right a2 of the centre; 0 outside the image), the right-hand side
 // b = f1 - f2 - a1 u0 + L(u0) with u0 over the full image, and x0 = image - u0.
-__device__ inline void cg_prepare_body(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
+__device__ __attribute__((always_inline)) inline void cg_prepare_body(const int *__restrict__ pts, int n, const int *__restrict__ lidx,
                                        const float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0,
                                        int nc, const double *__restrict__ a1, const double *__restrict__ a2,
                                        const double *__restrict__ f1, const double *__restrict__ F1,
@@ -856,7 +859,7 @@ struct StencilOf {
 
 // r0 = b - A x0 (FP64), d0 = r0, partials of |r0|^2
 template <int NC, typename T>
-__device__ inline void cg_init_body(const CGArgs<T> &a, const double *x0, int blk)
+__device__ __attribute__((always_inline)) inline void cg_init_body(const CGArgs<T> &a, const double *x0, int blk)
 {
     raise_wave_priority();
     int k = blk * blockDim.x + threadIdx.x;
@@ -914,7 +917,7 @@ struct Gather5 {
 // phase A of iteration it: stop test on |r_it|^2, d_it = r_it + beta d_{it-1} (it > 0),
 // partials of <r_it, A d_it>
 template <int NC, typename T>
-__device__ inline void cg_a_body(const CGArgs<T> &a, int it, int blk)
+__device__ __attribute__((always_inline)) inline void cg_a_body(const CGArgs<T> &a, int it, int blk)
 {
     raise_wave_priority();
     const int k = blk * blockDim.x + threadIdx.x;
@@ -1000,7 +1003,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_a(CGArgs<T> a, int it)
 // phase B of iteration it: alpha = |r_it|^2 / <r_it, A d_it>, x_{it+1} = x_it + alpha d_it,
 // r_{it+1} = b - A x_{it+1}, partials of |r_{it+1}|^2
 template <int NC, typename T>
-__device__ inline void cg_b_body(const CGArgs<T> &a, int it, int blk)
+__device__ __attribute__((always_inline)) inline void cg_b_body(const CGArgs<T> &a, int it, int blk)
 {
     raise_wave_priority();
     const int k = blk * blockDim.x + threadIdx.x;
@@ -1072,7 +1075,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_b(CGArgs<T> a, int it)
 
 // final colours: image = float(u0 + x_final) on hole points; sum of (prev - new)^2
 template <typename T>
-__device__ inline void poisson_write_body(float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0,
+__device__ __attribute__((always_inline)) inline void poisson_write_body(float *__restrict__ img, const float *__restrict__ orig, int Cs, int c0,
                                           int nc, const int *__restrict__ pts, int n, const T *x0, const T *x1,
                                           const int *__restrict__ done_at, double *__restrict__ partials, int nb,
                                           int blk)
