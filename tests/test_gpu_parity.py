@@ -318,6 +318,31 @@ def test_patchmatch_given_field_only_improves(ctx, oracle, syn):
     assert np.array_equal(n0, init) and bitexact(dd0[sel], d0[sel])
 
 
+@pytest.mark.parametrize("W,H", [(24, 40000), (65600, 24)])
+def test_patchmatch_tall_and_wide_images(ctx, oracle, syn, W, H):
+    """The candidate-pool sweep packs source coordinates into 16 bits each and holds at
+    most 16 random-search windows.  Rows 32768 and beyond (24 x 40000: pool sweep) must
+    unpack unsigned, and an image 65600 pixels wide (17 windows, a coordinate beyond 16
+    bits) must take the general sweep: every entry a source point, distances bit-exact."""
+    img = syn.texture(W, H, 3, 3, 11)
+    hole = np.zeros((H, W), np.uint8)
+    if H > W:
+        hole[33000:33012, 8:16] = 1
+        hole[100:104, 10:14] = 1
+    else:
+        hole[8:16, 40000:40040] = 1
+        hole[10:13, 100:104] = 1
+    ext, tgt, src, _ = oracle.scale_masks(hole, 9)
+    dp = oracle.dist_params("l2combined", 9, 10000.0, 0.05)
+    n, d = ctx.patchmatch("l2combined", img, src, img, tgt, None, seed=2, lam=0.05, iters=4, with_dist=True)
+    sel = tgt.astype(bool)
+    assert np.all(n[~sel] == -1)
+    xs, ys = n[sel][:, 0], n[sel][:, 1]
+    assert np.all((xs >= 0) & (xs < W) & (ys >= 0) & (ys < H))
+    assert np.all(src[ys, xs] == 1)
+    assert bitexact(d[sel], oracle.patch_distance_field(dp, img, img, n, tgt)[sel])
+
+
 def test_patchmatch_rgbd_and_invalid_init(ctx, oracle, syn):
     W, H = 120, 90
     img = np.concatenate([syn.texture(W, H, 3, 3, 7), syn.depth_planes(W, H, 5, 7)[..., None]], axis=2)
