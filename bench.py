@@ -162,7 +162,8 @@ def cpu_baseline(wl, gpu_iterations, threads):
 
 def cpu_full(wls, gpu_outs, threads, n):
     """Full reference runs (ImageInpainting::process to its stopping rule, OpenMP
-    PatchMatch) of the first n images, timed, and their hole PSNR beside the GPU's outputs
+    PatchMatch) of the first n images with their workload's parameters, timed, and their
+    hole PSNR (colour channels) and, for RGB-D, hole depth RMSE beside the GPU's outputs
     of the same images."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -173,14 +174,20 @@ def cpu_full(wls, gpu_outs, threads, n):
     for k in range(n):
         wl = wls[k]
         hole = wl.mask.astype(bool)
+        p = dict(wl.params, seed=1 + k, pm_threads=threads)
         with c_stdout_to_stderr():
             t0 = time.perf_counter()
-            out = ref.process(wl.image, wl.mask, cpu_params(O, threads, seed=1 + k))
+            out = ref.process(wl.image, wl.mask, O.InpaintParams(**p))
             sec = time.perf_counter() - t0
-        p_ref = psnr(out[hole], wl.truth[hole])
-        p_gpu = psnr(gpu_outs[k][hole], wl.truth[hole])
-        runs.append(dict(image=k, seconds=sec, mpix_s=wl.mask.size / 1e6 / sec, psnr_hole_db_reference=p_ref,
-                         psnr_hole_db_gpu=p_gpu, gpu_minus_reference_db=p_gpu - p_ref))
+        g = gpu_outs[k]
+        run = dict(image=k, seconds=sec, mpix_s=wl.mask.size / 1e6 / sec,
+                   psnr_hole_db_reference=psnr(out[hole][:, :3], wl.truth[hole][:, :3]),
+                   psnr_hole_db_gpu=psnr(g[hole][:, :3], wl.truth[hole][:, :3]))
+        run["gpu_minus_reference_db"] = run["psnr_hole_db_gpu"] - run["psnr_hole_db_reference"]
+        if wl.image.shape[2] == 4:
+            rm = lambda o: float(np.sqrt(np.mean((o[hole][:, 3].astype(np.float64) - wl.truth[hole][:, 3]) ** 2)))
+            run["depth_rmse_m_reference"], run["depth_rmse_m_gpu"] = rm(out), rm(g)
+        runs.append(run)
         log(f"full reference run {k}: {runs[-1]}")
     d = np.array([r["gpu_minus_reference_db"] for r in runs])
     return dict(threads=threads, runs=runs, mean_gpu_minus_reference_db=float(d.mean()),
@@ -233,7 +240,11 @@ def bench_rgbd(args, nli, syn):
                          launches=sweep["launches"], avg_launch_ms=sweep["ms"] / max(sweep["launches"], 1)),
         "cpu_baseline": None,
         "depth_rmse_m": float(np.mean(rmse)),
+        "cpu_full_runs": None,
     }
+    if args.cpu_full > 0:
+        log(f"full reference runs of {args.cpu_full} images on {args.cpu_threads} threads ...")
+        line["cpu_full_runs"] = cpu_full(wls, outs, args.cpu_threads, min(args.cpu_full, len(wls)))
     print(json.dumps(line), flush=True)
     for p in d_in + d_mask + d_out:
         ctx.dev_free(p)
