@@ -193,32 +193,55 @@ __device__ __attribute__((always_inline)) inline int nl_gather(bool live, int p,
     const int x_a = max(x - h, 0), x_b = min(x + h, W - 1);
     const int y_a = max(y - h, 0), y_b = min(y + h, H - 1);
     const int nh = y_b - y_a + 1, cnt = live ? (x_b - x_a + 1) * nh : 0;
+    // all loads first, from window pixels (the window's corner for elements past the
+    // window), so they travel together instead of one dependent trip per element
+    int ii[SLOTS], jj[SLOTS];
+    int2 c[SLOTS];
+    float cf[SLOTS];
+    bool hf[SLOTS];
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int e = gl + kNLG * s;
+        const bool in = e < cnt;
+        ii[s] = x_a + (in ? e / nh : 0);
+        jj[s] = y_a + (in ? e % nh : 0);
+        const size_t q = (size_t)W * jj[s] + ii[s];
+        c[s] = nnf[q];
+        hf[s] = conf && hole[q];
+        cf[s] = conf ? conf[q] : 1.0f;
+    }
     int m = 0;
 #pragma unroll
     for (int s = 0; s < SLOTS; s++) {
         const int e = gl + kNLG * s;
-        bool ok = e < cnt;
-        const int i = x_a + (ok ? e / nh : 0), j = y_a + (ok ? e % nh : 0);
-        const size_t q = (size_t)W * j + i;
-        int2 c = ok ? nnf[q] : make_int2(-1, -1);
-        ok = ok && c.x != -1;
-        c.x += x - i;
-        c.y += y - j;
-        ok = ok && c.x >= 0 && c.y >= 0 && c.x < W && c.y < H;
+        const int i = ii[s], j = jj[s];
+        int2 cc = c[s];
+        bool ok = e < cnt && cc.x != -1;
+        cc.x += x - i;
+        cc.y += y - j;
+        ok = ok && cc.x >= 0 && cc.y >= 0 && cc.x < W && cc.y < H;
         float v = 0.0f;
         if (ok) {
             v = wl[P * (j - y_a) + (i - x_a)];
-            if (conf && hole[q]) v *= conf[q];
+            if (hf[s]) v *= cf[s];
         }
         const uint32_t gb = (uint32_t)(__ballot(ok) >> (kNLG * grp)) & ((1u << kNLG) - 1);
         pos[s] = ok ? m + __popc(gb & ((1u << gl) - 1)) : -1;
         m += __popc(gb);
-        src[s] = ok ? W * c.y + c.x : 0;
+        src[s] = ok ? W * cc.y + cc.x : 0;
         wt[s] = v;
         if (ok) lw[pos[s]] = v;
     }
     wave_lds_sync();
     return m;
+}
+
+// Order key of a colour for std::sort's comparison: the float order as unsigned integers
+// (sign-magnitude folded), with -0 and +0 one key, as the float comparison has them equal.
+__device__ inline uint32_t order_key(float x)
+{
+    const uint32_t b = __float_as_uint(x + 0.0f);   // -0 + 0 = +0
+    return b ^ ((uint32_t)((int)b >> 31) | 0x80000000u);
 }
 
 // NL-means (patch_non_local_means.cpp:23-100): weighted average in gather order
@@ -232,8 +255,8 @@ __global__ __launch_bounds__(kBlock) void k_nlmeans_w(const float *__restrict__ 
 {
     constexpr int kE = kNLG * SLOTS;
     __shared__ float wl[kMaxP * kMaxP];
-    __shared__ float lw[kNLWaves][kNLPix][kE];
-    __shared__ float lv[kNLWaves][kNLPix][kMaxNC][kE];
+    __shared__ __align__(16) float lw[kNLWaves][kNLPix][kE];
+    __shared__ __align__(16) float lv[kNLWaves][kNLPix][kMaxNC][kE];
     for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
     __syncthreads();
     const int wave = (int)threadIdx.x / 64, lane = (int)threadIdx.x & 63;
@@ -245,19 +268,40 @@ __global__ __launch_bounds__(kBlock) void k_nlmeans_w(const float *__restrict__ 
         int src[SLOTS], pos[SLOTS];
         float wt[SLOTS];
         const int m = nl_gather<SLOTS>(live, pts[live ? k : base], W, H, P, hole, nnf, conf, wl, mw, src, wt, pos);
+        // every slot loads (non-contributors read pixel 0), then the contributors store
+        float val[SLOTS][kMaxNC];
+#pragma unroll
+        for (int s = 0; s < SLOTS; s++)
+#pragma unroll
+            for (int ch = 0; ch < kMaxNC; ch++) val[s][ch] = ch < nc ? img[(size_t)src[s] * Cs + c0 + ch] : 0.0f;
 #pragma unroll
         for (int s = 0; s < SLOTS; s++)
             if (pos[s] >= 0)
-                for (int ch = 0; ch < nc; ch++) lv[wave][grp][ch][pos[s]] = img[(size_t)src[s] * Cs + c0 + ch];
+#pragma unroll
+                for (int ch = 0; ch < kMaxNC; ch++)
+                    if (ch < nc) lv[wave][grp][ch][pos[s]] = val[s][ch];
         wave_lds_sync();
-        // the float sums in gather order, formed by every lane of the pixel alike
+        // the float sums in gather order, formed by every lane of the pixel alike, four
+        // contributors per LDS read
         float tw = 0.0f, tv[kMaxNC] = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int f = 0; f < m; f++) {
-            const float v = mw[f];
-            tw += v;
+        for (int f0 = 0; f0 < m; f0 += 4) {
+            const float4 w4 = *reinterpret_cast<const float4 *>(mw + f0);
+            float4 v4[kMaxNC];
 #pragma unroll
             for (int ch = 0; ch < kMaxNC; ch++)
-                if (ch < nc) tv[ch] += lv[wave][grp][ch][f] * v;
+                if (ch < nc) v4[ch] = *reinterpret_cast<const float4 *>(&lv[wave][grp][ch][f0]);
+            const float wv[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (f0 + j >= m) break;
+                const float v = wv[j];
+                tw += v;
+#pragma unroll
+                for (int ch = 0; ch < kMaxNC; ch++) {
+                    const float u[4] = {v4[ch].x, v4[ch].y, v4[ch].z, v4[ch].w};
+                    if (ch < nc) tv[ch] += u[j] * v;
+                }
+            }
         }
         if (live && gl == 0) {
             has[k] = tw > 0.0;
@@ -287,11 +331,12 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
                                                         float *__restrict__ newv, uint8_t *__restrict__ has)
 {
     constexpr int kE = kNLG * SLOTS;
+    static_assert(kE % 4 == 0, "contributor arrays are read four at a time");
     __shared__ float wl[kMaxP * kMaxP];
-    __shared__ float lw[kNLWaves][kNLPix][kE];
-    __shared__ float lv[kNLWaves][kNLPix][kE];
-    __shared__ float sv[kNLWaves][kNLPix][kE];
-    __shared__ float sw[kNLWaves][kNLPix][kE];
+    __shared__ __align__(16) float lw[kNLWaves][kNLPix][kE];
+    __shared__ __align__(16) float lv[kNLWaves][kNLPix][kE];
+    __shared__ __align__(16) float sv[kNLWaves][kNLPix][kE];
+    __shared__ __align__(16) float sw[kNLWaves][kNLPix][kE];
     __shared__ int sstk[kNLWaves][kNLPix][3 * kIntrosortStack];
     for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = w[i];
     __syncthreads();
@@ -305,23 +350,42 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
         float wt[SLOTS];
         const int m = nl_gather<SLOTS>(live, pts[live ? k : base], W, H, P, hole, nnf, conf, wl, mw, src, wt, pos);
         float tw = 0.0f;
-        for (int f = 0; f < m; f++) tw += mw[f];
+        for (int f0 = 0; f0 < m; f0 += 4) {
+            const float4 w4 = *reinterpret_cast<const float4 *>(mw + f0);
+            const float wv[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (f0 + j < m) tw += wv[j];
+        }
         const float half = tw / 2;
+        const int m4 = (m + 3) & ~3;
         for (int ch = 0; ch < nc; ch++) {
             float v[SLOTS];
 #pragma unroll
-            for (int s = 0; s < SLOTS; s++) {
-                v[s] = pos[s] >= 0 ? img[(size_t)src[s] * Cs + c0 + ch] : 0.0f;
+            for (int s = 0; s < SLOTS; s++) v[s] = img[(size_t)src[s] * Cs + c0 + ch];   // pixel 0 if not a contributor
+#pragma unroll
+            for (int s = 0; s < SLOTS; s++)
                 if (pos[s] >= 0) mv[pos[s]] = v[s];
-            }
+            if (gl < m4 - m) mv[m + gl] = INFINITY;   // padding: ranks below no finite colour
             wave_lds_sync();
+            // rank = the number of contributors before this one in std::sort's comparison
+            // (colour, then gather position): one 64-bit key comparison per pair
             int rank[SLOTS];
+            uint64_t key[SLOTS];
 #pragma unroll
-            for (int s = 0; s < SLOTS; s++) rank[s] = 0;
-            for (int f = 0; f < m; f++) {
-                const float u = mv[f];
+            for (int s = 0; s < SLOTS; s++) {
+                rank[s] = 0;
+                key[s] = ((uint64_t)order_key(v[s]) << 32) | (uint32_t)pos[s];
+            }
+            for (int f0 = 0; f0 < m4; f0 += 4) {
+                const float4 u4 = *reinterpret_cast<const float4 *>(mv + f0);
+                const float u[4] = {u4.x, u4.y, u4.z, u4.w};
 #pragma unroll
-                for (int s = 0; s < SLOTS; s++) rank[s] += (u < v[s] || (u == v[s] && f < pos[s])) ? 1 : 0;
+                for (int j = 0; j < 4; j++) {
+                    const uint64_t kf = ((uint64_t)order_key(u[j]) << 32) | (uint32_t)(f0 + j);
+#pragma unroll
+                    for (int s = 0; s < SLOTS; s++) rank[s] += kf < key[s] ? 1 : 0;
+                }
             }
 #pragma unroll
             for (int s = 0; s < SLOTS; s++)
@@ -337,21 +401,33 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
             float total = tw, med = 0.0f;
             const double unit = (double)tw * (0x1p-24 + 0x1p-50), halfd = (double)half;
             double S = (double)tw, S_prev = (double)tw;
-            int j_prev = 0, e = 0;
+            int j_prev = 0, e = m;
             bool tie = false, crossed = false;
-            for (; e < m; e++) {
-                if (e > 0 && ms[e] != ms[e - 1]) {
-                    S_prev = S;
-                    j_prev = e;
-                } else if (e > 0) {
-                    tie = true;
-                }
-                total -= mo[e];
-                S -= (double)mo[e];
-                if (total < half) {
-                    med = ms[e];
-                    crossed = true;
-                    break;
+            float prev = 0.0f;
+            // four sorted pairs per LDS read; the running totals step one pair at a time
+            for (int e0 = 0; e0 < m && !crossed; e0 += 4) {
+                const float4 a4 = *reinterpret_cast<const float4 *>(ms + e0);
+                const float4 b4 = *reinterpret_cast<const float4 *>(mo + e0);
+                const float av[4] = {a4.x, a4.y, a4.z, a4.w}, bv[4] = {b4.x, b4.y, b4.z, b4.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int ee = e0 + j;
+                    if (ee < m && !crossed) {
+                        if (ee > 0 && av[j] != prev) {
+                            S_prev = S;
+                            j_prev = ee;
+                        } else if (ee > 0) {
+                            tie = true;
+                        }
+                        total -= bv[j];
+                        S -= (double)bv[j];
+                        if (total < half) {
+                            med = av[j];
+                            crossed = true;
+                            e = ee;
+                        }
+                        prev = av[j];
+                    }
                 }
             }
             bool ok = !tie;
