@@ -862,39 +862,64 @@ __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMAr
         y = by + wave;
         return bx >= 0;
     };
-    // Lanes below kWaveTargets load their target's mask, match and distance here, so the
-    // loads travel with the tile's; a target is valid exactly when its distance in the
-    // wave's table is >= 0 (-1 otherwise), which the candidate generators below rely on.
-    bool my_valid = false;
-    int2 my_best = make_int2(-1, -1);
-    float my_bd = -1.0f;
-    if (lw < kWaveTargets) {
-        int x, y;
-        const bool tile = target_coords(lw, x, y);
-        const bool inb = tile && x >= r && y >= r && x < a.tW - r && y < a.tH - r;
-        const size_t p = inb ? (size_t)a.tW * y + x : 0;
-        const uint8_t m = a.tmask[p];
-        const int2 n = a.nnf_in[p];
-        const float d = a.dist_in[p];
-        my_valid = inb && m;
-        my_best = my_valid ? n : make_int2(-1, -1);
-        my_bd = my_valid ? d : -1.0f;
-    }
-    for (int i = threadIdx.x; i < P * P; i += blockDim.x) wl[i] = a.w[i];
-    for (int i = threadIdx.x; i < kTilesPerBlock * plane; i += blockDim.x) {
-        const int ti = i / plane, k = i % plane;
-        const int gx = tx0[ti] - r + k % LW, gy = ty0[ti] - r + k / LW;
-        const bool in = tx0[ti] >= 0 && (unsigned)gx < (unsigned)a.tW && (unsigned)gy < (unsigned)a.tH;
-        const size_t q = in ? (size_t)gy * a.tW + gx : 0;
-        float *px = tiles + (size_t)ti * tile_floats + (size_t)k * FP;
+    // Set-up loads, all issued before any is used (one memory round trip): lanes below
+    // kWaveTargets load their target's mask, match and distance; every thread loads the
+    // patch weight and its pixels of the tiles with their halo.  Addresses outside the
+    // images become element 0, whose value is then discarded.  A target is valid exactly
+    // when its distance in the wave's table is >= 0 (-1 otherwise), which the candidate
+    // generators below rely on.
+    auto tile_origin = [&](int ti, int &bx, int &by) {   // selected, not indexed
+        bx = tx0[0];
+        by = ty0[0];
 #pragma unroll
-        for (int c = 0; c < C; c++) px[c] = in ? a.tgt[q * C + c] : 0.0f;
+        for (int u = 1; u < kTilesPerBlock; u++)
+            if (ti == u) {
+                bx = tx0[u];
+                by = ty0[u];
+            }
+    };
+    int tgx, tgy;
+    const bool ttile = target_coords(lw % kWaveTargets, tgx, tgy);
+    const bool tinb = lw < kWaveTargets && ttile && tgx >= r && tgy >= r && tgx < a.tW - r && tgy < a.tH - r;
+    const size_t tp = tinb ? (size_t)a.tW * tgy + tgx : 0;
+    const uint8_t tm = a.tmask[tp];
+    const int2 tn = a.nnf_in[tp];
+    const float td = a.dist_in[tp];
+    const float wv = a.w[(int)threadIdx.x < P * P ? (int)threadIdx.x : 0];   // P * P <= 225 < kBlock
+    constexpr int kPix = (kTilesPerBlock * plane + kBlock - 1) / kBlock;       // tile pixels per thread
+    float pv[kPix][F];
+    bool pin[kPix];
+#pragma unroll
+    for (int ip = 0; ip < kPix; ip++) {
+        const int i = (int)threadIdx.x + ip * kBlock;
+        const int ti = i / plane, k = i % plane;
+        int bx, by;
+        tile_origin(ti, bx, by);
+        const int gx = bx - r + k % LW, gy = by - r + k / LW;
+        pin[ip] = i < kTilesPerBlock * plane && bx >= 0 && (unsigned)gx < (unsigned)a.tW &&
+                  (unsigned)gy < (unsigned)a.tH;
+        const size_t q = pin[ip] ? (size_t)gy * a.tW + gx : 0;
+#pragma unroll
+        for (int c = 0; c < C; c++) pv[ip][c] = a.tgt[q * C + c];
         if constexpr (GRAD) {
 #pragma unroll
-            for (int k2 = 0; k2 < 2 * C; k2++) px[C + k2] = in ? a.tgrad[q * 2 * C + k2] : 0.0f;
+            for (int k2 = 0; k2 < 2 * C; k2++) pv[ip][C + k2] = a.tgrad[q * 2 * C + k2];
         }
+    }
+    const bool my_valid = tinb && tm;
+    const int2 my_best = my_valid ? tn : make_int2(-1, -1);
+    const float my_bd = my_valid ? td : -1.0f;
+    if ((int)threadIdx.x < P * P) wl[threadIdx.x] = wv;
 #pragma unroll
-        for (int k2 = F; k2 < FP; k2++) px[k2] = 0.0f;
+    for (int ip = 0; ip < kPix; ip++) {
+        const int i = (int)threadIdx.x + ip * kBlock;
+        if (i < kTilesPerBlock * plane) {
+            float *px = tiles + (size_t)(i / plane) * tile_floats + (size_t)(i % plane) * FP;
+#pragma unroll
+            for (int c = 0; c < F; c++) px[c] = pin[ip] ? pv[ip][c] : 0.0f;
+#pragma unroll
+            for (int k2 = F; k2 < FP; k2++) px[k2] = 0.0f;
+        }
     }
     __syncthreads();
 
