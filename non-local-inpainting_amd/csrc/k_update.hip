@@ -905,24 +905,36 @@ struct CGArgs {
 };
 
 // Grid totals of the previous kernel's block partials part[c * nb + block]; every
-// thread of every block forms the same sums in the same order.  The first 4 partials
-// of each thread are loaded unconditionally-in-bounds up front so that the loads
-// overlap the caller's other loads.
+// thread of every block forms the same sums in the same order.  The first 4 partials of
+// each thread and channel are loaded together, in bounds (partial 0 stands in for those
+// past nb and is then discarded), so that they travel with the caller's other loads in
+// one memory round trip; grids of more than 4 * kBlock blocks add the rest afterwards.
 template <int NC>
 __device__ inline void load_partials(const double *part, int nb, double (&tot)[NC])
 {
+    double v[NC][4];
 #pragma unroll
-    for (int c = 0; c < NC; c++) {
-        const double *pc = part + (size_t)c * nb;
-        double v[4];
+    for (int c = 0; c < NC; c++)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int b = (int)threadIdx.x + j * kBlock;
-            v[j] = b < nb ? pc[b] : 0.0;
+            v[c][j] = part[(size_t)c * nb + (b < nb ? b : 0)];
         }
-        double t = ((v[0] + v[1]) + v[2]) + v[3];
-        for (int b = (int)threadIdx.x + 4 * kBlock; b < nb; b += kBlock) t += pc[b];
-        tot[c] = t;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        double u[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) u[j] = (int)threadIdx.x + j * kBlock < nb ? v[c][j] : 0.0;
+        tot[c] = ((u[0] + u[1]) + u[2]) + u[3];
+    }
+    if (nb > 4 * kBlock) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const double *pc = part + (size_t)c * nb;
+            double t = tot[c];
+            for (int b = (int)threadIdx.x + 4 * kBlock; b < nb; b += kBlock) t += pc[b];
+            tot[c] = t;
+        }
     }
 }
 
@@ -969,8 +981,9 @@ __global__ void k_cg_init(CGArgs<T> a, const double *x0)
 // block 0 set it in the same kernel sees the channel stopped, which is what its own
 // stop test says.
 
-// Loads of one unknown and its four stencil neighbours from two vectors (u, v);
-// issued before the grid totals are formed so that both memory round trips overlap.
+// Loads of one unknown and its four stencil neighbours from two vectors (u, v), all
+// issued together (a missing neighbour reads the unknown itself and is then zeroed), before
+// the grid totals are formed so that the memory round trips overlap.
 template <int NC, typename T>
 struct Gather5 {
     T u[NC][5], v[NC][5];
@@ -982,11 +995,19 @@ struct Gather5 {
             const size_t o = (size_t)c * n;
 #pragma unroll
             for (int j = 0; j < 5; j++) {
-                const bool ok = idx[j] >= 0;
-                u[c][j] = (with_u && ok) ? uv[o + idx[j]] : (T)0;
-                v[c][j] = ok ? vv[o + idx[j]] : (T)0;
+                const size_t q = o + (idx[j] >= 0 ? idx[j] : k);
+                v[c][j] = vv[q];
+                u[c][j] = with_u ? uv[q] : (T)0;   // with_u is uniform
             }
         }
+#pragma unroll
+        for (int c = 0; c < NC; c++)
+#pragma unroll
+            for (int j = 1; j < 5; j++)
+                if (idx[j] < 0) {
+                    u[c][j] = (T)0;
+                    v[c][j] = (T)0;
+                }
     }
 };
 
@@ -1000,20 +1021,24 @@ __device__ __attribute__((always_inline)) inline void cg_a_body(const CGArgs<T> 
     const bool live = k < a.n;
     const T *dp = a.d[it & 1];
     T *dn = a.d[(it + 1) & 1];
-    const int4 nb = live ? a.nbr[k] : make_int4(-1, -1, -1, -1);
-    Stencil s64{};
-    if (live) s64 = a.st[k];
-    double tot[NC];
-    load_partials<NC>(a.prr, a.nb, tot);
+    // first memory round trip: this unknown's neighbours and stencil, the partials and the
+    // CG history (in-bounds indices everywhere, values of the stand-ins discarded)
+    const int kc = live ? k : 0;
+    const int4 nb0 = a.nbr[kc];
+    const Stencil s64 = a.st[kc];
+    const int4 nb = live ? nb0 : make_int4(-1, -1, -1, -1);
     int done[NC];
     double h0[NC], hprev[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         done[c] = a.done_at[c];
         const double *h = a.hist + (size_t)c * a.hist_len;
-        h0[c] = it > 0 ? h[0] : 0.0;
-        hprev[c] = it > 0 ? h[it - 1] : 0.0;
+        h0[c] = h[0];
+        hprev[c] = h[it > 0 ? it - 1 : 0];
     }
+    double tot[NC];
+    load_partials<NC>(a.prr, a.nb, tot);
+    // second: the neighbours' values
     Gather5<NC, T> g;   // u = r, v = d_{it-1}
     if (live) g.load(a.r, dp, a.n, k, nb, it > 0);
     block_sum<NC>(tot);
@@ -1087,19 +1112,21 @@ __device__ __attribute__((always_inline)) inline void cg_b_body(const CGArgs<T> 
     const T *xp = a.x[it & 1];
     T *xn = a.x[(it + 1) & 1];
     const T *dcur = a.d[(it + 1) & 1];
-    const int4 nb = live ? a.nbr[k] : make_int4(-1, -1, -1, -1);
-    Stencil s64{};
-    if (live) s64 = a.st[k];
-    double tot[NC];
-    load_partials<NC>(a.prq, a.nb, tot);
+    // first memory round trip: neighbours, stencil, right-hand side, partials, history
+    const int kc = live ? k : 0;
+    const int4 nb0 = a.nbr[kc];
+    const Stencil s64 = a.st[kc];
+    const int4 nb = live ? nb0 : make_int4(-1, -1, -1, -1);
     int done[NC];
     double hit[NC], bk[NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
         done[c] = a.done_at[c];
         hit[c] = a.hist[(size_t)c * a.hist_len + it];
-        bk[c] = live ? a.b[(size_t)c * a.n + k] : 0.0;
+        bk[c] = a.b[(size_t)c * a.n + kc];   // (unused unless live)
     }
+    double tot[NC];
+    load_partials<NC>(a.prq, a.nb, tot);
     Gather5<NC, T> g;   // u = x_it, v = d_it
     if (live) g.load(xp, dcur, a.n, k, nb, true);
     block_sum<NC>(tot);
