@@ -522,12 +522,12 @@ __device__ __attribute__((always_inline)) inline void poisson_coef_rows(int qx, 
             bool ok[CH];
             int2 nb[CH];
             float cf[CH];
+            uint8_t ex[CH];
 #pragma unroll
-            for (int j = 0; j < CH; j++) {
+            for (int j = 0; j < CH; j++) {   // every flag, entry and confidence of the row first
                 const int px = px0 + j;
-                const bool in = px <= x_hi;
-                const size_t pidx = row + (in ? px : x_lo);
-                ok[j] = in && ext[pidx];
+                const size_t pidx = row + (px <= x_hi ? px : x_lo);
+                ex[j] = ext[pidx];
                 nb[j] = nnf[pidx];
                 cf[j] = conf ? conf[pidx] : 1.0f;
             }
@@ -536,14 +536,17 @@ __device__ __attribute__((always_inline)) inline void poisson_coef_rows(int qx, 
             for (int j = 0; j < CH; j++) {
                 const int px = px0 + j;
                 const int n_x = nb[j].x + qx - px, n_y = nb[j].y + qy - py;
-                ok[j] = ok[j] && n_x >= 0 && n_y >= 0 && n_x < W && n_y < H;
+                ok[j] = px <= x_hi && ex[j] && n_x >= 0 && n_y >= 0 && n_x < W && n_y < H;
                 const bool right = n_x + 1 < W, below = n_y + 1 < H;
                 const float *iv = img + (ok[j] ? (size_t)W * n_y + n_x : 0) * Cs + c0;
+                // in-bounds loads for every centre (the sample itself stands in for a
+                // missing neighbour)
+                const size_t ro = (ok[j] && right) ? (size_t)Cs : 0, bo = (ok[j] && below) ? (size_t)W * Cs : 0;
 #pragma unroll
                 for (int ch = 0; ch < NC; ch++) {
                     const float c = iv[ch];
-                    const float r = (ok[j] && right) ? iv[ch + Cs] : 0.0f;
-                    const float d = (ok[j] && below) ? iv[ch + (size_t)W * Cs] : 0.0f;
+                    const float r = (ok[j] && right) ? iv[ch + ro] : 0.0f;
+                    const float d = (ok[j] && below) ? iv[ch + bo] : 0.0f;
                     v[j][ch] = c;
                     gx[j][ch] = right ? r - c : 0.0f;
                     gy[j][ch] = below ? d - c : 0.0f;
@@ -667,13 +670,15 @@ __device__ __attribute__((always_inline)) inline void poisson_coef16_body(const 
     int2 nb[SLOTS];
     float cf[SLOTS];
     int wi[SLOTS];
+    uint8_t ex[SLOTS];
 #pragma unroll
     for (int s = 0; s < SLOTS; s++) {
         const int e = gl + kNLG * s;
         const bool in = e < cnt;
         const int py = y_lo + (in ? e / nw : 0), px = x_lo + (in ? e % nw : 0);
         const size_t pidx = (size_t)W * py + px;
-        ok[s] = in && ext[pidx];
+        ex[s] = ext[pidx];
+        ok[s] = in;
         nb[s] = nnf[pidx];
         cf[s] = conf ? conf[pidx] : 1.0f;
         nb[s].x += qx - px;
@@ -684,14 +689,16 @@ __device__ __attribute__((always_inline)) inline void poisson_coef16_body(const 
 #pragma unroll
     for (int s = 0; s < SLOTS; s++) {
         const int n_x = nb[s].x, n_y = nb[s].y;
-        ok[s] = ok[s] && n_x >= 0 && n_y >= 0 && n_x < W && n_y < H;
+        ok[s] = ok[s] && ex[s] && n_x >= 0 && n_y >= 0 && n_x < W && n_y < H;
         const bool right = n_x + 1 < W, below = n_y + 1 < H;
         const float *iv = img + (ok[s] ? (size_t)W * n_y + n_x : 0) * Cs + c0;
+        // in-bounds loads for every slot (the sample itself stands in for a missing neighbour)
+        const size_t ro = (ok[s] && right) ? (size_t)Cs : 0, bo = (ok[s] && below) ? (size_t)W * Cs : 0;
 #pragma unroll
         for (int ch = 0; ch < NC; ch++) {
             const float c = iv[ch];
-            const float r = (ok[s] && right) ? iv[ch + Cs] : 0.0f;
-            const float d = (ok[s] && below) ? iv[ch + (size_t)W * Cs] : 0.0f;
+            const float r = (ok[s] && right) ? iv[ch + ro] : 0.0f;
+            const float d = (ok[s] && below) ? iv[ch + bo] : 0.0f;
             v[s][ch] = c;
             v[s][NC + ch] = right ? r - c : 0.0f;
             v[s][2 * NC + ch] = below ? d - c : 0.0f;
