@@ -318,7 +318,8 @@ def main():
     ap.add_argument("--no-breakdown", action="store_true", help="skip the all-kernel timing batch")
     ap.add_argument("--no-colour", action="store_true", help="skip the sRGB <-> Lab conversion timing")
     ap.add_argument("--sweep-sample", type=int, default=16, metavar="K",
-                    help="time one PatchMatch sweep in K with HIP events (roofline); 1 = every sweep")
+                    help="time one PatchMatch call in K with HIP events (its sweeps as one span; "
+                         "roofline); 1 = every call")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run 3 device-to-device copies of MIB MiB (the known-size copy that "
                          "calibrates hardware byte counters)")
@@ -370,8 +371,9 @@ def main():
         ctx.inpaint_batch_dev(d_in, d_mask, d_out, Ws, Hs, C, params, args.workers)
         log(f"warmup {i}: {time.perf_counter() - t0:.2f} s for {len(wls)} images")
 
-    # timed steps: only the dominant kernel (the PatchMatch sweep) carries timing events,
-    # one sweep in --sweep-sample of them (an event pair is a marker in the stream)
+    # timed steps: only the dominant kernel (the PatchMatch sweep) carries timing events:
+    # the sweeps of one PatchMatch call in --sweep-sample, as one span (an event pair is a
+    # marker in the stream)
     ctx.profile(["pm_sweep"], period=args.sweep_sample)
     ctx.lib.nli_synchronize(ctx.h)
     t0 = time.perf_counter()
@@ -389,11 +391,13 @@ def main():
     ms_per_step = elapsed / args.steps * 1e3
     value = len(wls) * W * H / 1e6 / (ms_per_step / 1e3)
 
-    # roofline of the dominant kernel.  In the timed steps the event pair of a sampled sweep
-    # brackets that launch on its own stream, but 7 other drivers' kernels share the CUs
+    # roofline of the dominant kernel.  In the timed steps the event pair of a sampled call
+    # brackets its sweeps on their own stream, but 7 other drivers' kernels share the CUs
     # meanwhile, so the span is not the kernel's duration.  The kernel's own duration comes
-    # from one image on one driver, every sweep timed (the configuration a kernel trace of
-    # one driver reports); the timed region's figure is kept beside it.
+    # from one image on one driver, every call's sweeps timed as one span (the
+    # configuration a kernel trace of one driver reports; the span holds the back-to-back
+    # sweeps and their dispatch, not a marker per sweep); the timed region's figure is kept
+    # beside it.
     ctx.profile(["pm_sweep"], period=1)
     ctx.inpaint_dev(d_in[0], d_mask[0], d_out[0], W, H, C, params)
     sw1 = ctx.profile_read("pm_sweep")
@@ -404,13 +408,13 @@ def main():
                 peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS, traffic=measured_traffic(),
                 traffic_source="profiles/k_pm_sweep_rgb1024_nlpoisson_traffic.md (bytes per sweep)",
                 launches=sw1["launches"], avg_launch_ms=sw1["ms"] / max(sw1["launches"], 1),
-                run="image 0 of the batch on one driver after the timed steps, same parameters, every sweep "
-                    "event-timed on its stream",
+                run="image 0 of the batch on one driver after the timed steps, same parameters, the 10 "
+                    "sweeps of every PatchMatch call event-timed as one span on their stream",
                 timed_region=dict(achieved=shared, frac=shared / FP32_PEAK_TFLOPS, launches=sweep["launches"],
                                   avg_launch_ms=sweep["ms"] / max(sweep["launches"], 1),
-                                  sampled=f"one sweep in {args.sweep_sample} (in issue order over all drivers) "
-                                          f"carries the event pair; spans include the other drivers' "
-                                          f"concurrent kernels"),
+                                  sampled=f"one PatchMatch call in {args.sweep_sample} (in issue order over "
+                                          f"all drivers) carries an event pair around its sweeps; spans "
+                                          f"include the other drivers' concurrent kernels"),
                 note="bound by FP32/FP64 vector (VALU) work, no MFMA: patch distances are not a contraction; "
                      "peak = the FP32 vector peak; algorithmic flops T*(2+R)*P^2*(9C+4) per sweep")
 

@@ -1525,15 +1525,25 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     const uint64_t tune_key = ((uint64_t)d.kind << 56) | ((uint64_t)C << 48) | ((uint64_t)d.patch << 40) |
                               (uint64_t)(32 - __builtin_clz((unsigned)pr.ntiles | 1u));
     if (pool && tpb == 0) tpb = tuned_tiles_per_block(tune_key);
-    for (int k = 0; k < K; k++) {
+    auto set_sweep = [&](int k) {
         a.nnf_in = n_cur;
         a.dist_in = d_cur;
         a.nnf_out = n_oth;
         a.dist_out = d_oth;
         a.step = s0 >> (k % (levels + 1));
         a.stream = (uint32_t)k;
-        if (pool && tpb == 0) tpb = tune_tiles_per_block(tune_key, launch_pool, s);
-        prof::begin(TK_PM_SWEEP, s);
+    };
+    // tuning runs on the first sweep's input (its outputs are overwritten by that sweep),
+    // outside the timed span
+    if (K > 0 && pool && tpb == 0) {
+        set_sweep(0);
+        tpb = tune_tiles_per_block(tune_key, launch_pool, s);
+    }
+    // the K sweeps are timed as one span (one event pair: a pair per sweep would add its
+    // markers' dispatch to every sweep)
+    if (K > 0) prof::begin(TK_PM_SWEEP, s);
+    for (int k = 0; k < K; k++) {
+        set_sweep(k);
         if (pool) {
             launch_pool(tpb);
         } else {
@@ -1542,10 +1552,10 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
 #undef NLI_SWEEP
             NLI_LAUNCH_CHECK();
         }
-        prof::end(TK_PM_SWEEP, s, sweep_flops, pr.targets);
         std::swap(n_cur, n_oth);
         std::swap(d_cur, d_oth);
     }
+    if (K > 0) prof::end(TK_PM_SWEEP, s, sweep_flops * K, (double)pr.targets * K, K);
 }
 
 // ---- several PatchMatch calls in lockstep ----
@@ -1654,12 +1664,10 @@ void patchmatch_batch(const PMJob *jobs, int n, PMBatchWork &bw, hipStream_t s)
     const uint64_t tune_key = ((uint64_t)d0.kind << 56) | ((uint64_t)C << 48) | ((uint64_t)P << 40) |
                               (uint64_t)(32 - __builtin_clz((unsigned)total_tiles | 1u));
     if (tpb == 0) tpb = tuned_tiles_per_block(tune_key);
-    for (int k = 0; k < K; k++) {
-        if (tpb == 0) tpb = tune_tiles_per_block(tune_key, [&](int t) { launch(t, 0); }, s);
-        prof::begin(TK_PM_SWEEP, s);
-        launch(tpb, k);
-        prof::end(TK_PM_SWEEP, s, sweep_flops, targets);
-    }
+    if (K > 0 && tpb == 0) tpb = tune_tiles_per_block(tune_key, [&](int t) { launch(t, 0); }, s);
+    if (K > 0) prof::begin(TK_PM_SWEEP, s);   // the K sweeps as one timed span
+    for (int k = 0; k < K; k++) launch(tpb, k);
+    if (K > 0) prof::end(TK_PM_SWEEP, s, sweep_flops * K, targets * K, K);
 }
 
 // ---- fixed-field distances without allocation ----

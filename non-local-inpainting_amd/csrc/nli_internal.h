@@ -100,9 +100,10 @@ namespace prof {
 // kernel is timed (launches counted in issue order over all streams)
 void enable(unsigned kernels, unsigned period = 1);
 // brackets launches on stream s; `work` is the algorithmic flops or bytes of the
-// bracketed launches, `units` the elements they process
+// bracketed launches, `units` the elements they process, `launches` their number (a
+// bracket of several back-to-back launches of one kernel: one event pair for all)
 void begin(int kernel, hipStream_t s);
-void end(int kernel, hipStream_t s, double work, double units);
+void end(int kernel, hipStream_t s, double work, double units, int launches = 1);
 // synchronises and returns launches, total milliseconds, total work, total units
 void read(int kernel, int *launches, double *ms, double *work, double *units);
 }  // namespace prof

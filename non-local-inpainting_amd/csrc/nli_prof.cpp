@@ -15,6 +15,7 @@ struct Rec {
     hipStream_t stream;
     hipEvent_t a, b;
     double work, units;
+    int launches;
 };
 
 struct State {
@@ -53,7 +54,7 @@ void drain(State &s)
         NLI_HIP(hipEventSynchronize(r.b));
         float ms = 0;
         NLI_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-        s.launches[r.kernel] += 1;
+        s.launches[r.kernel] += r.launches;
         s.ms[r.kernel] += ms;
         s.work[r.kernel] += r.work;
         s.units[r.kernel] += r.units;
@@ -90,12 +91,12 @@ void begin(int kernel, hipStream_t stream)
     t_sampled[kernel] = s.seen[kernel].fetch_add(1, std::memory_order_relaxed) % s.period.load() == 0;
     if (!t_sampled[kernel]) return;
     std::lock_guard<std::mutex> g(s.mu);
-    Rec r{kernel, stream, take(s), take(s), 0, 0};
+    Rec r{kernel, stream, take(s), take(s), 0, 0, 0};
     NLI_HIP(hipEventRecord(r.a, stream));
     s.open.push_back(r);
 }
 
-void end(int kernel, hipStream_t stream, double work, double units)
+void end(int kernel, hipStream_t stream, double work, double units, int launches)
 {
     State &s = st();
     if (!((s.mask.load(std::memory_order_relaxed) >> kernel) & 1u) || !t_sampled[kernel]) return;
@@ -108,6 +109,7 @@ void end(int kernel, hipStream_t stream, double work, double units)
             NLI_HIP(hipEventRecord(r.b, stream));
             r.work = work;
             r.units = units;
+            r.launches = launches;
             s.done.push_back(r);
             if (s.done.size() > 4096) drain(s);
             return;

@@ -49,9 +49,10 @@ def test_plain_dropin_rejects_rgbd_distance_on_rgb(ctx, nli):
 
 
 def test_kernel_timing_all_and_sampled(ctx, syn, nli):
-    """nli_profile_enable(_sampled): with every sweep timed, every sweep of the run is
-    counted; timing one sweep in 4 counts every 4th of the same launches (the first, the
-    fifth, ...); the algorithmic work of the timed sweeps is positive."""
+    """nli_profile_enable(_sampled): a PatchMatch call's sweeps are timed as one span (one
+    event pair, its sweeps counted as launches).  With every call timed, every sweep of
+    the run is counted; timing one call in 4 counts the sweeps of every 4th call (the
+    first, the fifth, ...); the algorithmic work of the timed sweeps is positive."""
     img = syn.texture(160, 128, 3, 4, 2)
     hole = syn.ellipse_holes(160, 128, 3, 0.12, 2)
     p = nli.Inpainting(method="nlpoisson", patch=9, iters=3, scales=2, coarse=0.5, cg_tol=1e-4, cg_limit=50, seed=2)
@@ -65,8 +66,10 @@ def test_kernel_timing_all_and_sampled(ctx, syn, nli):
     ctx.profile(False)
     print(f"sweeps timed: all {full['launches']}, one in 4 {part['launches']}; "
           f"average {full['ms'] / full['launches'] * 1e3:.1f} vs {part['ms'] / part['launches'] * 1e3:.1f} us")
-    assert full["launches"] > 8 and full["work"] > 0
-    assert part["launches"] == (full["launches"] + 3) // 4
+    K = p.pm_iters   # sweeps per PatchMatch call
+    assert full["launches"] > 8 and full["work"] > 0 and full["launches"] % K == 0
+    calls = full["launches"] // K
+    assert part["launches"] == (calls + 3) // 4 * K
     assert part["work"] > 0
     with pytest.raises(ValueError):
         ctx.profile(["pm_sweep"], period=0)
