@@ -885,7 +885,8 @@ __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMAr
     const uint8_t tm = a.tmask[tp];
     const int2 tn = a.nnf_in[tp];
     const float td = a.dist_in[tp];
-    const float wv = a.w[(int)threadIdx.x < P * P ? (int)threadIdx.x : 0];   // P * P <= 225 < kBlock
+    static_assert(P * P <= kBlock, "one thread per patch weight");
+    const float wv = a.w[(int)threadIdx.x < P * P ? (int)threadIdx.x : 0];
     constexpr int kPix = (kTilesPerBlock * plane + kBlock - 1) / kBlock;       // tile pixels per thread
     float pv[kPix][F];
     bool pin[kPix];
