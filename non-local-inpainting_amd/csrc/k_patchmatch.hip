@@ -781,26 +781,32 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
                 }
             } else {
                 // colour-only distances: float accumulation in row-major order, as
-                // l2_norm_patch_distance.cpp / l1_norm_patch_distance.cpp
+                // l2_norm_patch_distance.cpp / l1_norm_patch_distance.cpp; two patch rows
+                // per pass (a row's arithmetic costs about as much as the pool's
+                // bookkeeping of a pass), the bound tested after each row
 #pragma unroll
-                for (int dx = 0; dx < P; dx++) {
-                    float norm = 0.0f;
+                for (int h = 0; h < 2; h++) {
+                    if (!active) break;
 #pragma unroll
-                    for (int c = 0; c < C; c++) {
-                        float d = row[dx * C + c] - tg.v(c, dx, dy);
-                        float term = (KIND == DIST_L1) ? fabsf(d) : d * d;
-                        norm = (c == 0) ? term : norm + term;
+                    for (int dx = 0; dx < P; dx++) {
+                        float norm = 0.0f;
+#pragma unroll
+                        for (int c = 0; c < C; c++) {
+                            float d = row[dx * C + c] - tg.v(c, dx, dy);
+                            float term = (KIND == DIST_L1) ? fabsf(d) : d * d;
+                            norm = (c == 0) ? term : norm + term;
+                        }
+                        facc += wl[dy * P + dx] * norm;
                     }
-                    facc += wl[dy * P + dx] * norm;
-                }
-                if (facc > bound * (float)PP * (1.0f + 0x1p-20f)) {
-                    active = false;
-                } else if (dy + 1 >= P) {
-                    record(facc / (float)PP);
-                    active = false;
-                } else {
-                    row += stride;
-                    dy++;
+                    if (facc > bound * (float)PP * (1.0f + 0x1p-20f)) {
+                        active = false;
+                    } else if (dy + 1 >= P) {
+                        record(facc / (float)PP);
+                        active = false;
+                    } else {
+                        row += stride;
+                        dy++;
+                    }
                 }
             }
         }
