@@ -438,6 +438,12 @@ __global__ __launch_bounds__(kBlock) void k_nlmedians_w(const float *__restrict_
                     tie = true;
                 }
                 ok = !tie || (tw > 0.0f && S_prev - j_prev * unit >= halfd && S + j * unit < halfd);
+                // equal colours that differ in their bits (-0.0 and +0.0) leave the
+                // median's bits to the order within the group: then std::sort's order
+                if (ok && tie) {
+                    const uint32_t bits = __float_as_uint(ms[e]);
+                    for (int i = j_prev; i < j && ok; i++) ok = __float_as_uint(ms[i]) == bits;
+                }
             }
             // otherwise std::sort's order itself (the same decision in every lane of the pixel)
             if (!ok && gl == 0) {

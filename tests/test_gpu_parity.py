@@ -389,6 +389,25 @@ def test_nlmedians_real_fields_bitexact(ctx, oracle, syn, seed, P, C):
     assert ta == tb
 
 
+def test_nlmedians_signed_zeros_and_ties(ctx, oracle, syn):
+    """Colours quantised to a few levels, zeros stored as -0.0 or +0.0 at random: the
+    reference's comparison (a.color < b.color) holds -0.0 and +0.0 equal, so they tie and
+    std::sort's order decides which one the median lands on; the result must carry the
+    same sign bit as the oracle's, bit for bit."""
+    img, hole, ext, nnf, conf = field_case(oracle, syn, kind="l1", seed=6)
+    q = (np.round((img - np.float32(0.5)) * np.float32(4)) / np.float32(4)).astype(np.float32)
+    rng = np.random.default_rng(6)
+    q[(q == 0) & (rng.random(q.shape) < 0.5)] = np.float32(-0.0)
+    zeros = q[q == 0]
+    assert np.signbit(zeros).any() and (~np.signbit(zeros)).any()
+    a, ta = ctx.update("nlmedians", q, q, hole, ext, nnf, conf)
+    b, tb = oracle.update("nlmedians", q, q, hole, ext, nnf, conf)
+    assert bitexact(a, b)
+    assert ta == tb
+    sel = hole.astype(bool)
+    assert np.signbit(a[sel][a[sel] == 0]).any()   # some medians land on a -0.0
+
+
 def test_nlmedians(ctx, oracle, syn):
     img, hole, ext, nnf, conf = field_case(oracle, syn, kind="l1")
     a, ta = ctx.update("nlmedians", img, img, hole, ext, nnf, conf)
