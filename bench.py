@@ -160,9 +160,9 @@ def cpu_baseline(wl, gpu_iterations, threads):
                        f"iterations per scale x {ref_s / port_s:.3f}")
 
 
-def cpu_full(wls, gpu_outs, threads, n):
+def cpu_full(wls, gpu_outs, threads, n, start=0):
     """Full reference runs (ImageInpainting::process to its stopping rule, OpenMP
-    PatchMatch) of the first n images with their workload's parameters, timed, and their
+    PatchMatch) of n images from index start with their workload's parameters, timed, and their
     hole PSNR (colour channels) and, for RGB-D, hole depth RMSE beside the GPU's outputs
     of the same images."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -171,7 +171,7 @@ def cpu_full(wls, gpu_outs, threads, n):
     omp_threads(threads)
     ref = O.Reference(REF_OMP)
     runs = []
-    for k in range(n):
+    for k in range(start, start + n):
         wl = wls[k]
         hole = wl.mask.astype(bool)
         p = dict(wl.params, seed=1 + k, pm_threads=threads)
@@ -244,7 +244,7 @@ def bench_rgbd(args, nli, syn):
     }
     if args.cpu_full > 0:
         log(f"full reference runs of {args.cpu_full} images on {args.cpu_threads} threads ...")
-        line["cpu_full_runs"] = cpu_full(wls, outs, args.cpu_threads, min(args.cpu_full, len(wls)))
+        line["cpu_full_runs"] = cpu_full(wls, outs, args.cpu_threads, args.cpu_full, args.cpu_full_start)
     print(json.dumps(line), flush=True)
     for p in d_in + d_mask + d_out:
         ctx.dev_free(p)
@@ -327,6 +327,9 @@ def main():
     ap.add_argument("--cpu-full", type=int, default=0, metavar="N",
                     help="also run the reference's CPU path in full on the first N images (minutes each) and "
                          "compare hole PSNR with the GPU's outputs")
+    ap.add_argument("--cpu-full-start", type=int, default=0, metavar="S",
+                    help="with --cpu-full N: run images S .. S+N-1 instead (--images must cover them), so "
+                         "several calls can cover different images")
     ap.add_argument("--fixed-iters", type=int, default=10, metavar="K",
                     help="also time the batch, one image and the plain drop-in with exactly K inpainting "
                          "iterations per scale (stopping tolerance 0), for like-for-like comparisons; 0 = skip")
@@ -335,6 +338,10 @@ def main():
                          "sides 128-1024, nlmeans/nlmedians, patch 7/9, 8 scales (per-image parameters); "
                          "rgbd1080: 1920x1080 RGB-D images, nldepth (lambda_rgb 1, lambda_d 0), patch 9, 11 scales")
     args = ap.parse_args()
+    if args.cpu_full > 0 and args.cpu_full_start + args.cpu_full > args.images:
+        print(f"--cpu-full-start {args.cpu_full_start} + --cpu-full {args.cpu_full} exceeds --images "
+              f"{args.images}", file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (images are independent: "
               f"one process per GPU would take its share of them)", file=sys.stderr)
@@ -523,7 +530,7 @@ def main():
     full = None
     if args.cpu_full > 0:
         log(f"full reference runs of {args.cpu_full} images on {args.cpu_threads} threads ...")
-        full = cpu_full(wls, outs, args.cpu_threads, min(args.cpu_full, len(wls)))
+        full = cpu_full(wls, outs, args.cpu_threads, args.cpu_full, args.cpu_full_start)
 
     line = {
         "metric": METRIC, "value": value, "unit": "MPix/s", "n_gpus": 1, "steps": args.steps,
