@@ -222,10 +222,12 @@ int nli_patchmatch_dev(nli_ctx *ctx, const nli_dist_params *dist, const nli_pm_p
                        uint64_t seed, int32_t *out_nnf, float *out_dist);
 
 /* Tile arrangement of the PatchMatch sweep, process-wide: 0 = autotuned (the default:
- * the first call for each distance kind, channel count, patch side and size class times
- * one sweep in each arrangement and keeps the faster), 1 or 2 target tiles per
- * workgroup = forced.  Fields and distances do not depend on the arrangement.  No
- * counterpart in the reference. */
+ * for each distance kind, channel count, patch side, row-scan setting and size class, one
+ * call times sweeps in each arrangement and keeps the fastest); forced: 1 or 2 target
+ * tiles per workgroup, or 3 = one tile per workgroup with two waves per tile row (half
+ * the targets per wave; runs as 1 where the row scan is on, and in lockstep batches).
+ * Fields and distances do not depend on the arrangement.  No counterpart in the
+ * reference. */
 int nli_pm_tiles_per_block(int tiles);
 
 /* Row scan of the PatchMatch pool sweep, process-wide: 0 = automatic (the default: on for

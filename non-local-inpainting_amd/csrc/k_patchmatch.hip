@@ -617,7 +617,8 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
 }
 
 // ---- sweep with per-wave candidate pools (compile-time patch) ----
-// A block takes one or two target tiles; wave w takes row w of them (16 or 32 targets).
+// A block takes one or two target tiles; wave w takes row w of them (16 or 32 targets), or,
+// in the split-row arrangement, one tile with two waves per row (8 targets a wave).
 // The sweep's candidates -- the 4 jump-flood neighbours' shifted matches, then, around the
 // resulting best, the reference's random search (windows max(W,H)/2^k, at most `shots`
 // tries, the first in-source draw kept) -- are first written to LDS per wave and then
@@ -632,9 +633,16 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep(PMArgs a)
 // the same candidates (neighbours in order left, right, up, down; windows largest first).
 constexpr int kMaxR = 16;                      // random-search windows (max(W, H) < 65536)
 constexpr int kWaves = kBlock / 64;
-constexpr int kMaxTilesPerBlock = 2;   // tile arrangements the autotuner chooses from: 1 or 2 per block
 static_assert(kWaves == TY, "wave w covers row w of each tile");
-// a block takes `tpb` tiles, so a wave takes tpb * TX targets
+// Tile arrangements (the autotuner chooses among them; fields do not depend on the choice):
+//   1: one tile per block of kBlock threads, wave w takes tile row w (TX targets);
+//   2: two tiles per block, wave w takes row w of both (2 TX targets);
+//   3: one tile per block of 2 kBlock threads, waves 2w and 2w + 1 take the two halves of
+//      tile row w (TX / 2 targets each): twice the waves per tile, each with half the
+//      work, for scales whose tiles leave most of the GPU idle.  The row scan needs whole
+//      tile rows in one wave, so this arrangement runs only with the row scan off.
+// In the kernels TPB is the tiles per block and WPR the waves per tile row.
+constexpr int kArrangements = 3;
 
 __device__ inline int lane_rank(uint64_t mask)
 {
@@ -671,9 +679,9 @@ __host__ __device__ inline size_t pool_tables_offset(int F, int P, int tpb)
 }
 // per wave: two keys (u64), best (int2), distance (float) per target; candidate (int)
 // and list entry (u16) per task
-__host__ __device__ inline size_t pool_wave_bytes(int R, int tpb)
+__host__ __device__ inline size_t pool_wave_bytes(int R, int tpb, int wpr)
 {
-    const size_t targets = (size_t)tpb * TX;
+    const size_t targets = (size_t)tpb * TX / wpr;
     return targets * (2 * sizeof(uint64_t) + sizeof(int2) + sizeof(float)) +
            targets * pool_tasks_per_target(R) * (sizeof(int) + sizeof(uint16_t));
 }
@@ -700,11 +708,13 @@ struct WavePool {
 // search) is dropped as soon as keys[target] lies below the target's distance: then
 // propagation has improved the target and its random search runs again around the new
 // best (sweep_pool_body), so these results would be discarded.
-template <int KIND, int C, int P, int FP, bool LAST_FIRST, int TPB>
+template <int KIND, int C, int P, int FP, bool LAST_FIRST, int TPB, int WPR>
 __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const float *tiles, int LW, int tile_floats,
                          int wave, const WavePool &wp, int nvalid)
 {
-    constexpr int kWaveTargets = TPB * TX;
+    constexpr int kSeg = TX / WPR;               // targets of one tile row in this wave
+    constexpr int kWaveTargets = TPB * kSeg;
+    const int wrow = wave / WPR, wcol = (wave % WPR) * kSeg;
     constexpr bool GRAD = Feat<KIND>::grad;
     constexpr int r = P / 2;
     constexpr unsigned PP = (unsigned)(P * P);
@@ -759,7 +769,7 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
         if (active) {
             bound = fminf(bound, key_dist(wp.keys[kslot]));
             if (kslot != t && key_dist(wp.keys[t]) < bd_t) bound = -1.0f;   // superseded: the first check drops it
-            const TgtLdsI<FP> tg{tiles + (t / TX) * tile_floats, LW, t % TX, wave};
+            const TgtLdsI<FP> tg{tiles + (t / kSeg) * tile_floats, LW, wcol + t % kSeg, wrow};
             if constexpr (GRAD) {
                 // one patch row per pass: row dy is in A, row dy + 1 goes to B, then B -> A
                 float B[P + 1][C];
@@ -813,11 +823,14 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
     }
 }
 
-template <int KIND, int C, int PS, int TPB>
+template <int KIND, int C, int PS, int TPB, int WPR>
 __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMArgs &a, int slot)
 {
+    static_assert(WPR == 1 || (WPR == 2 && TPB == 1), "arrangements: TPB tiles of one wave a row, or one tile of two");
     constexpr int kTilesPerBlock = TPB;
-    constexpr int kWaveTargets = TPB * TX;
+    constexpr int NT = kBlock * WPR;             // threads per block
+    constexpr int kSeg = TX / WPR;               // targets of one tile row in a wave
+    constexpr int kWaveTargets = TPB * kSeg;
     static_assert(PS > 0 && (KIND == DIST_L2RGBD || KIND == DIST_L2 || KIND == DIST_L1 ||
                              (KIND == DIST_L2COMBINED && (C == 1 || C == 3))),
                   "pool sweep: compile-time patch; gradient distances need the on-the-fly source gradient");
@@ -836,9 +849,11 @@ __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMAr
     const int ntask = pool_tasks_per_target(R);
     const int wave = (int)threadIdx.x / 64;
     const int lw = (int)threadIdx.x & 63;
+    const int wrow = wave / WPR, wcol = (wave % WPR) * kSeg;   // this wave's tile row and first column
     WavePool wp;
     {
-        char *pb = reinterpret_cast<char *>(plds) + pool_tables_offset(F, P, TPB) + wave * pool_wave_bytes(R, TPB);
+        char *pb =
+            reinterpret_cast<char *>(plds) + pool_tables_offset(F, P, TPB) + wave * pool_wave_bytes(R, TPB, WPR);
         wp.keys = reinterpret_cast<uint64_t *>(pb);
         wp.best = reinterpret_cast<int2 *>(pb + 2 * kWaveTargets * sizeof(uint64_t));
         wp.bd = reinterpret_cast<float *>(pb + kWaveTargets * (2 * sizeof(uint64_t) + sizeof(int2)));
@@ -854,18 +869,18 @@ __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMAr
         tx0[ti] = tile >= 0 ? (tile % a.ntiles_x) * TX : -1;
         ty0[ti] = tile >= 0 ? (tile / a.ntiles_x) * TY : -1;
     }
-    // targets of this wave: t = 16 * tile + column, row `wave` of the tile
+    // targets of this wave: t = kSeg * tile + i, column wcol + i of tile row wrow
     // (returns whether target t's tile exists)
     auto target_coords = [&](int t, int &x, int &y) {
         int bx = tx0[0], by = ty0[0];   // selected, not indexed: the arrays stay in registers
 #pragma unroll
         for (int ti = 1; ti < kTilesPerBlock; ti++)
-            if (t / TX == ti) {
+            if (t / kSeg == ti) {
                 bx = tx0[ti];
                 by = ty0[ti];
             }
-        x = bx + t % TX;
-        y = by + wave;
+        x = bx + wcol + t % kSeg;
+        y = by + wrow;
         return bx >= 0;
     };
     // Set-up loads, all issued before any is used (one memory round trip): lanes below
@@ -891,14 +906,14 @@ __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMAr
     const uint8_t tm = a.tmask[tp];
     const int2 tn = a.nnf_in[tp];
     const float td = a.dist_in[tp];
-    static_assert(P * P <= kBlock, "one thread per patch weight");
+    static_assert(P * P <= NT, "one thread per patch weight");
     const float wv = a.w[(int)threadIdx.x < P * P ? (int)threadIdx.x : 0];
-    constexpr int kPix = (kTilesPerBlock * plane + kBlock - 1) / kBlock;       // tile pixels per thread
+    constexpr int kPix = (kTilesPerBlock * plane + NT - 1) / NT;       // tile pixels per thread
     float pv[kPix][F];
     bool pin[kPix];
 #pragma unroll
     for (int ip = 0; ip < kPix; ip++) {
-        const int i = (int)threadIdx.x + ip * kBlock;
+        const int i = (int)threadIdx.x + ip * NT;
         const int ti = i / plane, k = i % plane;
         int bx, by;
         tile_origin(ti, bx, by);
@@ -919,7 +934,7 @@ __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMAr
     if ((int)threadIdx.x < P * P) wl[threadIdx.x] = wv;
 #pragma unroll
     for (int ip = 0; ip < kPix; ip++) {
-        const int i = (int)threadIdx.x + ip * kBlock;
+        const int i = (int)threadIdx.x + ip * NT;
         if (i < kTilesPerBlock * plane) {
             float *px = tiles + (size_t)(i / plane) * tile_floats + (size_t)(i % plane) * FP;
 #pragma unroll
@@ -1052,7 +1067,7 @@ __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMAr
     }
     wave_sync();
     // the list is evaluated from its end: propagation first, then the windows smallest first
-    run_pool<KIND, C, P, FP, true, TPB>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);
+    run_pool<KIND, C, P, FP, true, TPB, WPR>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);
     wave_sync();
     bool again = false;
     if (lw < kWaveTargets) {
@@ -1075,7 +1090,7 @@ __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMAr
     if (redo) {
         nvalid = gen_random(redo, 0);
         wave_sync();
-        run_pool<KIND, C, P, FP, true, TPB>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);
+        run_pool<KIND, C, P, FP, true, TPB, WPR>(a, wl, tiles, LW, tile_floats, wave, wp, nvalid);
         wave_sync();
         if (lw < kWaveTargets) {
             const uint64_t kr = wp.keys[kWaveTargets + lw];
@@ -1094,8 +1109,9 @@ __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMAr
     // sweeps).  Within each 16-target tile row of this wave, the targets repeatedly try
     // the shifted match of their scan-side neighbour whenever that neighbour improved in
     // the previous round, until no target improves: an improvement travels the whole tile
-    // row in one sweep (the rows lie in one wave, so no workgroup barrier is needed).
-    if (a.row_scan) {
+    // row in one sweep (the rows lie in one wave, so no workgroup barrier is needed; the
+    // host never runs the split-row arrangement with the scan on).
+    if (WPR == 1 && a.row_scan) {
         const int sh = (a.stream & 1u) ? 1 : -1;   // the neighbour at x + sh proposes its match - sh
         uint64_t changed = ~0ull;                  // round 0: every neighbour proposes
         for (int round = 0; round < TX; round++) {
@@ -1115,7 +1131,7 @@ __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMAr
             if (m == 0) break;
             if (ok) wp.list[lane_rank(m)] = (uint16_t)lw;
             wave_sync();
-            run_pool<KIND, C, P, FP, false, TPB>(a, wl, tiles, LW, tile_floats, wave, wp, __popcll(m));
+            run_pool<KIND, C, P, FP, false, TPB, WPR>(a, wl, tiles, LW, tile_floats, wave, wp, __popcll(m));
             wave_sync();
             bool improved = false;
             if (lw < kWaveTargets) {
@@ -1149,10 +1165,10 @@ __device__ inline int xcd_slot()
     return (int)(blockIdx.x % kXCDs) * per + (int)(blockIdx.x / kXCDs);
 }
 
-template <int KIND, int C, int PS, int TPB>
-__global__ __launch_bounds__(kBlock) void k_pm_sweep_pool(PMArgs a)
+template <int KIND, int C, int PS, int TPB, int WPR>
+__global__ __launch_bounds__(kBlock * WPR) void k_pm_sweep_pool(PMArgs a)
 {
-    sweep_pool_body<KIND, C, PS, TPB>(a, xcd_slot());
+    sweep_pool_body<KIND, C, PS, TPB, WPR>(a, xcd_slot());
 }
 
 // batched sweep k: start[i] is the first slot (block of TPB tiles) of image i; the slots of
@@ -1174,7 +1190,7 @@ __global__ __launch_bounds__(kBlock) void k_pm_sweep_pool_batch(const PMArgs *__
     a.dist_out = odd ? a.dist_pp[0] : a.dist_pp[1];
     a.step = a.s0 >> (k % (a.levels + 1));
     a.stream = (uint32_t)k;
-    sweep_pool_body<KIND, C, PS, TPB>(a, slot - start[img]);
+    sweep_pool_body<KIND, C, PS, TPB, 1>(a, slot - start[img]);
 }
 
 template <int KIND, int C, int PS>
@@ -1366,7 +1382,7 @@ namespace {
 // Choices are kept per process and device for (kind, channels, patch, log2 tile count).
 std::mutex g_tune_mu;
 std::map<std::pair<int, uint64_t>, int> g_tuned;
-std::atomic<int> g_forced_tpb{0};
+std::atomic<int> g_forced_tpb{0};   // forced tile arrangement (0: autotuned)
 std::atomic<int> g_row_scan{0};   // 0: automatic (RGB-D distances), 1: on, 2: off
 
 int current_device()
@@ -1374,6 +1390,20 @@ int current_device()
     int dev = 0;
     NLI_HIP(hipGetDevice(&dev));
     return dev;
+}
+
+// compute units of the current device (looked up once per device)
+int device_cus()
+{
+    static std::mutex mu;
+    static std::map<int, int> cus;
+    const int dev = current_device();
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cus.find(dev);
+    if (it != cus.end()) return it->second;
+    int n = 0;
+    NLI_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    return cus[dev] = n;
 }
 
 // The tile arrangement of a call: the tuned one, or 1 (one tile per workgroup) until the
@@ -1384,7 +1414,7 @@ int current_device()
 // 813x813 scale still tuned to two tiles, 14 % slower in its later sweeps).
 constexpr int kTuneCall = 8;
 std::map<std::pair<int, uint64_t>, int> g_tune_calls;
-int tuned_tiles_per_block(uint64_t key)
+int tuned_arrangement(uint64_t key)
 {
     std::lock_guard<std::mutex> g(g_tune_mu);
     const auto k = std::make_pair(current_device(), key);
@@ -1393,41 +1423,47 @@ int tuned_tiles_per_block(uint64_t key)
     return ++g_tune_calls[k] >= kTuneCall ? 0 : 1;
 }
 
-// Times sweeps in each arrangement (the sweep is deterministic, so running it again on
-// the same input writes the same output) and keeps the faster.
+// Times sweeps in each of the n arrangements arr[] (the sweep is deterministic, so running
+// it again on the same input writes the same output) and keeps the fastest: each
+// arrangement twice, in mirrored order (1 2 2 1, 1 2 3 3 2 1, ...), so that none gains
+// from running after another (caches, clocks); the smallest total wins.
 template <class Launch>
-int tune_tiles_per_block(uint64_t key, Launch &&launch, hipStream_t s)
+int tune_arrangement(uint64_t key, const int *arr, int n, Launch &&launch, hipStream_t s)
 {
-    // both arrangements twice, in the order 1 2 2 1, so that neither gains from running
-    // after the other (caches, clocks); the smaller total wins
-    static_assert(kMaxTilesPerBlock == 2, "the tuning order assumes two arrangements");
-    constexpr int order[4] = {1, 2, 2, 1};
-    hipEvent_t ev[5];
-    for (hipEvent_t &e : ev) NLI_HIP(hipEventCreate(&e));
+    if (n < 1 || n > kArrangements) throw std::logic_error("tile arrangements: 1 to 3");
+    const int runs = 2 * n;
+    int order[2 * kArrangements];
+    for (int i = 0; i < n; i++) order[i] = order[runs - 1 - i] = arr[i];
+    hipEvent_t ev[2 * kArrangements + 1];
+    for (int i = 0; i <= runs; i++) NLI_HIP(hipEventCreate(&ev[i]));
     NLI_HIP(hipEventRecord(ev[0], s));
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < runs; i++) {
         launch(order[i]);
         NLI_HIP(hipEventRecord(ev[i + 1], s));
     }
-    NLI_HIP(hipEventSynchronize(ev[4]));
-    float total[3] = {0.0f, 0.0f, 0.0f};
-    for (int i = 0; i < 4; i++) {
+    NLI_HIP(hipEventSynchronize(ev[runs]));
+    float total[kArrangements + 1] = {};
+    for (int i = 0; i < runs; i++) {
         float ms = 0.0f;
         NLI_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
         total[order[i]] += ms;
     }
-    const int best = total[2] < total[1] ? 2 : 1;
-    for (hipEvent_t e : ev) NLI_HIP(hipEventDestroy(e));
+    int best = arr[0];
+    for (int i = 1; i < n; i++)
+        if (total[arr[i]] < total[best]) best = arr[i];
+    for (int i = 0; i <= runs; i++) NLI_HIP(hipEventDestroy(ev[i]));
     std::lock_guard<std::mutex> g(g_tune_mu);
     return g_tuned.emplace(std::make_pair(current_device(), key), best).first->second;
 }
 
 }  // namespace
 
-void set_pm_tiles_per_block(int tpb)
+void set_pm_tiles_per_block(int arrangement)
 {
-    if (tpb < 0 || tpb > kMaxTilesPerBlock) throw std::invalid_argument("tiles per block: 0 (auto), 1 or 2");
-    g_forced_tpb = tpb;
+    if (arrangement < 0 || arrangement > kArrangements)
+        throw std::invalid_argument("tile arrangement: 0 (auto), 1 (one tile per workgroup), 2 (two tiles per "
+                                    "workgroup) or 3 (one tile, two waves per tile row)");
+    g_forced_tpb = arrangement;
 }
 int pm_tiles_per_block() { return g_forced_tpb.load(); }
 
@@ -1507,13 +1543,16 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
                                      (d.kind == DIST_L2RGBD && C == 4 && d.patch == 9) ||
                                      ((d.kind == DIST_L2 || d.kind == DIST_L1) && C == 3 &&
                                       (d.patch == 7 || d.patch == 9)));
-    // launches one candidate-pool sweep with `tpb` tiles per block
-    auto launch_pool = [&](int tpb) {
-        const size_t lds_pool = pool_tables_offset(F, d.patch, tpb) + kWaves * pool_wave_bytes(R, tpb);
+    // launches one candidate-pool sweep in tile arrangement `arr` (1, 2 or 3)
+    auto launch_pool = [&](int arr) {
+        const int tpb = arr == 2 ? 2 : 1, wpr = arr == 3 ? 2 : 1;
+        const size_t lds_pool =
+            pool_tables_offset(F, d.patch, tpb) + (size_t)kWaves * wpr * pool_wave_bytes(R, tpb, wpr);
         const int blocks = ((pr.ntiles + tpb - 1) / tpb + kXCDs - 1) / kXCDs * kXCDs;
-#define NLI_POOL(KD, CC, PP)                                                                  \
-    (tpb == 1 ? k_pm_sweep_pool<KD, CC, PP, 1> : k_pm_sweep_pool<KD, CC, PP, 2>)                \
-        <<<blocks, kBlock, lds_pool, s>>>(a)
+#define NLI_POOL(KD, CC, PP)                                                                          \
+    (arr == 2 ? k_pm_sweep_pool<KD, CC, PP, 2, 1>                                                     \
+              : (arr == 3 ? k_pm_sweep_pool<KD, CC, PP, 1, 2> : k_pm_sweep_pool<KD, CC, PP, 1, 1>))     \
+        <<<blocks, kBlock * wpr, lds_pool, s>>>(a)
         if (d.kind == DIST_L2RGBD) NLI_POOL(DIST_L2RGBD, 4, 9);
         else if (d.kind == DIST_L2 && d.patch == 7) NLI_POOL(DIST_L2, 3, 7);
         else if (d.kind == DIST_L2) NLI_POOL(DIST_L2, 3, 9);
@@ -1525,13 +1564,22 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
 #undef NLI_POOL
         NLI_LAUNCH_CHECK();
     };
-    // tile arrangement: forced, tuned earlier for this (kind, channels, patch, size), the
-    // default before tuning, or tuned now on the first sweep (both arrangements give
-    // identical fields)
-    int tpb = pool ? pm_tiles_per_block() : 0;
+    // tile arrangement: forced, tuned earlier for this (kind, channels, patch, row scan,
+    // size class), the default before tuning, or tuned now on the first sweep (every
+    // arrangement gives identical fields).  The split-row arrangement 3 needs the row scan
+    // off (forced 3 with the scan on runs arrangement 1), and it pays only where the tiles
+    // leave compute units idle (1024x1024 workload: 7-12 % faster sweeps at 72-144 tiles,
+    // slower from 200 tiles, 26 % slower at 3008), so the tuner is offered it up to 3/4
+    // of the compute units
+    const bool split_ok = !a.row_scan && (int64_t)pr.ntiles * 4 <= (int64_t)device_cus() * 3;
+    int arr_list[kArrangements] = {1, 2}, n_arr = 2;
+    if (split_ok) arr_list[n_arr++] = 3;
+    int arr = pool ? pm_tiles_per_block() : 0;
+    if (arr == 3 && a.row_scan) arr = 1;
     const uint64_t tune_key = ((uint64_t)d.kind << 56) | ((uint64_t)C << 48) | ((uint64_t)d.patch << 40) |
+                              ((uint64_t)(split_ok ? 1 : 0) << 34) | ((uint64_t)(a.row_scan ? 1 : 0) << 32) |
                               (uint64_t)(32 - __builtin_clz((unsigned)pr.ntiles | 1u));
-    if (pool && tpb == 0) tpb = tuned_tiles_per_block(tune_key);
+    if (pool && arr == 0) arr = tuned_arrangement(tune_key);
     auto set_sweep = [&](int k) {
         a.nnf_in = n_cur;
         a.dist_in = d_cur;
@@ -1542,9 +1590,9 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     };
     // tuning runs on the first sweep's input (its outputs are overwritten by that sweep),
     // outside the timed span
-    if (K > 0 && pool && tpb == 0) {
+    if (K > 0 && pool && arr == 0) {
         set_sweep(0);
-        tpb = tune_tiles_per_block(tune_key, launch_pool, s);
+        arr = tune_arrangement(tune_key, arr_list, n_arr, launch_pool, s);
     }
     // the K sweeps are timed as one span (one event pair: a pair per sweep would add its
     // markers' dispatch to every sweep)
@@ -1552,7 +1600,7 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     for (int k = 0; k < K; k++) {
         set_sweep(k);
         if (pool) {
-            launch_pool(tpb);
+            launch_pool(arr);
         } else {
 #define NLI_SWEEP(KD, CC, PP) k_pm_sweep<KD, CC, PP><<<sweep_blocks, kBlock, lds, s>>>(a)
             NLI_DISPATCH(d.kind, C, d.patch, NLI_SWEEP);
@@ -1651,7 +1699,7 @@ void patchmatch_batch(const PMJob *jobs, int n, PMBatchWork &bw, hipStream_t s)
     auto launch = [&](int tpb, int k) {
         const int *start = tpb == 1 ? dst + (n + 1) : dst + 2 * (n + 1);
         const int slots = tpb == 1 ? st1[n] : st2[n];
-        const size_t lds_pool = pool_tables_offset(F, P, tpb) + kWaves * pool_wave_bytes(maxR, tpb);
+        const size_t lds_pool = pool_tables_offset(F, P, tpb) + kWaves * pool_wave_bytes(maxR, tpb, 1);
         const int blocks = (slots + kXCDs - 1) / kXCDs * kXCDs;
 #define NLI_POOL_B(KD, CC, PP)                                                                        \
     (tpb == 1 ? k_pm_sweep_pool_batch<KD, CC, PP, 1> : k_pm_sweep_pool_batch<KD, CC, PP, 2>)            \
@@ -1667,11 +1715,15 @@ void patchmatch_batch(const PMJob *jobs, int n, PMBatchWork &bw, hipStream_t s)
 #undef NLI_POOL_B
         NLI_LAUNCH_CHECK();
     };
+    // tile arrangement 1 or 2: the lockstep kernels take whole tile rows per wave (a forced
+    // split-row arrangement 3 runs as 1); batched calls tune under keys of their own
     int tpb = pm_tiles_per_block();
+    if (tpb > 2) tpb = 1;
     const uint64_t tune_key = ((uint64_t)d0.kind << 56) | ((uint64_t)C << 48) | ((uint64_t)P << 40) |
-                              (uint64_t)(32 - __builtin_clz((unsigned)total_tiles | 1u));
-    if (tpb == 0) tpb = tuned_tiles_per_block(tune_key);
-    if (K > 0 && tpb == 0) tpb = tune_tiles_per_block(tune_key, [&](int t) { launch(t, 0); }, s);
+                              (1ull << 33) | (uint64_t)(32 - __builtin_clz((unsigned)total_tiles | 1u));
+    if (tpb == 0) tpb = tuned_arrangement(tune_key);
+    static constexpr int kBatchArrangements[2] = {1, 2};
+    if (K > 0 && tpb == 0) tpb = tune_arrangement(tune_key, kBatchArrangements, 2, [&](int t) { launch(t, 0); }, s);
     if (K > 0) prof::begin(TK_PM_SWEEP, s);   // the K sweeps as one timed span
     for (int k = 0; k < K; k++) launch(tpb, k);
     if (K > 0) prof::end(TK_PM_SWEEP, s, sweep_flops * K, targets * K, K);

@@ -467,7 +467,7 @@ int nli_copy_d2h(nli_ctx *ctx, void *dst, const void *src, uint64_t bytes)
 
 int nli_pm_tiles_per_block(int tiles)
 {
-    if (tiles < 0 || tiles > 2) return NLI_E_ARGUMENT;
+    if (tiles < 0 || tiles > 3) return NLI_E_ARGUMENT;
     nli::set_pm_tiles_per_block(tiles);
     return NLI_OK;
 }
