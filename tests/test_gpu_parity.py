@@ -274,9 +274,10 @@ def test_patchmatch_energy_1024(ctx, oracle, syn):
 @pytest.mark.parametrize("kind,C,patch", [("l2combined", 3, 9), ("l2combined", 3, 11), ("l2", 3, 7),
                                            ("l1", 3, 9), ("l2rgbd", 4, 9)])
 def test_pool_tile_arrangements_give_identical_fields(ctx, syn, nli, kind, C, patch, row_scan):
-    """The autotuner chooses between 1 and 2 target tiles per workgroup; the fields and
-    distances must not depend on the choice (with the row scan automatic, and forced on
-    for every distance kind)."""
+    """The autotuner chooses among the tile arrangements (1 or 2 target tiles per
+    workgroup, or one tile with two waves per tile row where the row scan is off); the
+    fields and distances must not depend on the choice (with the row scan automatic, and
+    forced on for every distance kind: forced arrangement 3 then runs as 1)."""
     W, H = 150, 118
     img = syn.texture(W, H, 3, 5, 9)
     if C == 4:
@@ -286,7 +287,7 @@ def test_pool_tile_arrangements_give_identical_fields(ctx, syn, nli, kind, C, pa
     res = []
     assert ctx.lib.nli_pm_row_scan(row_scan) == 0
     try:
-        for tiles in (1, 2, 0):
+        for tiles in (1, 2, 3, 0):
             assert ctx.lib.nli_pm_tiles_per_block(tiles) == 0
             n, d = ctx.patchmatch(kind, img, src, img, tgt, None, seed=4, patch=patch, lam=0.3, lam_d=0.6,
                                   with_dist=True)
@@ -600,6 +601,38 @@ def test_process_psnr_vs_reference_run(ctx, nli, name):
             assert np.array_equal(out[~sel], img[~sel])
         g.append(psnr(out[sel], img[sel]))
     assert_psnr_within_0p1db(name, g, ref)
+
+
+def test_process_psnr_1024_vs_reference_runs(ctx, nli, syn):
+    """The 1024x1024 nlpoisson workload (images synthetic.config_1024(seed=1+k), the bench's
+    parameters with FP64 CG): whole GPU runs against whole runs of the reference's own
+    process() (serial build, srand(seed): tests/golden/make_process_psnr_1024.py, about 35
+    minutes a run).  Per image the mean hole PSNR over 4 GPU seeds is compared with the
+    mean over the reference's seeds; over the images the mean difference must not lie more
+    than two standard errors below -0.1 dB.  One run varies by several tenths of a dB from
+    seed to seed at this size and reference runs are few, so the bar is one-sided here (the
+    GPU is not worse); it is resolved in both directions at the smaller sizes above."""
+    g = golden("process_psnr_1024")
+    images, ref = g["images"], g["psnr"]   # psnr[image, seed]; nan where not run
+    n_ref = np.sum(np.isfinite(ref), axis=1)
+    keep = n_ref > 0
+    assert keep.sum() >= 4, "too few reference runs in the fixture"
+    kw = dict(method="nlpoisson", patch=9, sigma=10000.0, lam=0.05, iters=200, tol=0.1, scales=11, coarse=0.1,
+              init="poisson", cg_tol=1e-4, cg_limit=100)
+    diffs = []
+    for k, r in zip(images[keep], ref[keep]):
+        w = syn.config_1024(seed=1 + int(k))
+        sel = w.mask.astype(bool)
+        gp = []
+        for seed in range(1, 5):
+            out, _ = ctx.inpaint(w.image, w.mask, nli.Inpainting(seed=seed, **kw))
+            gp.append(psnr(out[sel], w.truth[sel]))
+        diffs.append(np.mean(gp) - np.nanmean(r))
+    d = np.array(diffs)
+    se = d.std(ddof=1) / np.sqrt(len(d))
+    print(f"1024x1024: GPU - reference hole PSNR per image {np.round(d, 3)}; mean {d.mean():+.3f} dB, "
+          f"SE {se:.3f} dB over {len(d)} images ({int(n_ref.sum())} reference runs)")
+    assert d.mean() - 2 * se >= -0.1
 
 
 def test_process_psnr_rgb_pipeline_vs_reference_run(ctx, nli):
