@@ -82,3 +82,26 @@ def test_null_context_is_rejected(nli):
     lib = nli.load_library()
     assert lib.nli_synchronize(None) == nli.NLI_E_ARGUMENT
     assert lib.nli_poisson_fill(None, None, 0, 0, 0) == nli.NLI_E_ARGUMENT
+
+
+def test_process_wide_knobs_check_their_range(nli):
+    """The sweep's tile arrangement (0 tuned, 1-3 forced), the row scan (0-2) and the batch
+    mode (0-2) are process-wide settings: out-of-range values are refused, in-range ones
+    accepted (no GPU work is done)."""
+    lib = nli.load_library()
+    try:
+        for v in (0, 1, 2, 3):
+            assert lib.nli_pm_tiles_per_block(v) == nli.NLI_OK
+        for v in (-1, 4):
+            assert lib.nli_pm_tiles_per_block(v) == nli.NLI_E_ARGUMENT
+        for v in (0, 1, 2):
+            assert lib.nli_pm_row_scan(v) == nli.NLI_OK
+        assert lib.nli_pm_row_scan(3) == nli.NLI_E_ARGUMENT
+        for v in (0, 1, 2):
+            assert lib.nli_batch_mode(v) == nli.NLI_OK
+        for v in (-1, 3):
+            assert lib.nli_batch_mode(v) == nli.NLI_E_ARGUMENT
+    finally:
+        lib.nli_pm_tiles_per_block(0)
+        lib.nli_pm_row_scan(0)
+        lib.nli_batch_mode(0)
