@@ -650,20 +650,32 @@ def test_process_psnr_rgb_pipeline_vs_reference_run(ctx, nli):
     assert_psnr_within_0p1db("rgb_nlpoisson128", g, ref)
 
 
-def test_process_rgbd(ctx, oracle, syn, nli):
-    import oracle as O
-    W, H = 120, 96
-    img = np.concatenate([syn.texture(W, H, 3, 3, 4), syn.depth_planes(W, H, 5, 4)[..., None]], axis=2)
-    hole = syn.stripe_holes(W, H, 3, 6, 4)
-    kw = dict(method="nldepth", patch=7, iters=4, scales=3, coarse=0.4, lambda_rgb=1.0, lambda_d=0.0, seed=2)
-    out, st = ctx.inpaint(img, hole, nli.Inpainting(**kw))
-    ref, _ = oracle.process(img, hole, O.InpaintParams(**kw))
+def test_process_rgbd_vs_reference_runs(ctx, nli):
+    """nldepth with the 1920x1080 RGB-D workload's weights (lambda_rgb 1, lambda_d 0) on a
+    128x96 RGB + depth image with stripe holes: whole GPU runs against whole runs of the
+    reference's own process() (serial build, srand(seed), seeds 1-256:
+    tests/golden/make_process_psnr.py).  The colour channels' hole PSNR and the depth
+    channel's (20 log10(10 m / RMSE)) must each reach the reference's within 0.1 dB on
+    means over many seeds (one run varies by 0.2 dB in colour and 0.3 dB in depth from
+    seed to seed).  Known pixels are unchanged."""
+    g = golden("process_psnr")
+    ref_c, ref_d = g["rgbd_nldepth128"], g["rgbd_nldepth128_depth"]
+    img, hole, kw, _ = process_case("rgbd_nldepth128")
     sel = hole.astype(bool)
-    assert np.all(np.isfinite(out))
-    err_g = np.sqrt(np.mean((out[sel][:, 3] - img[sel][:, 3]) ** 2))
-    err_o = np.sqrt(np.mean((ref[sel][:, 3] - img[sel][:, 3]) ** 2))
-    print(f"nldepth depth RMSE GPU {err_g:.4f} m, reference algorithm {err_o:.4f} m")
-    assert err_g <= 1.5 * err_o + 0.01
+    gc, gd = [], []
+    for seed in range(1, RGBD_PROCESS_SEEDS + 1):
+        out, st = ctx.inpaint(img, hole, nli.Inpainting(seed=seed, **kw))
+        if seed == 1:
+            assert np.array_equal(out[~sel], img[~sel])
+        gc.append(psnr(out[sel][:, :3], img[sel][:, :3]))
+        gd.append(psnr(out[sel][:, 3] / 10.0, img[sel][:, 3] / 10.0))
+    assert_psnr_within_0p1db("rgbd_nldepth128 colour", gc, ref_c)
+    assert_psnr_within_0p1db("rgbd_nldepth128 depth", gd, ref_d)
+
+
+# GPU seeds of the RGB-D case: two standard errors of the depth difference stay near
+# 0.065 dB with the reference's 256 seeds (seed-to-seed spread 0.32 dB)
+RGBD_PROCESS_SEEDS = 160
 
 
 def test_batch_equals_single_images(ctx, syn, nli):
