@@ -133,9 +133,19 @@ __device__ inline bool is_target(const PMArgs &a, int x, int y)
     return x >= r && y >= r && x < a.tW - r && y < a.tH - r && a.tmask[(size_t)a.tW * y + x];
 }
 
+// L2RGBD with lambda_rgb = 1 and lambda_d = 0 (the RGB-D workload's weights), a
+// pool-sweep kernel variant only: the colour-gradient and depth-value terms then enter
+// the distance multiplied by exactly 0, so the distance is the colour-value sum plus the
+// depth-gradient sum, accumulated as in the general kernel (l2_rgbd_patch_distance.cpp:
+// 114-116 with those weights) -- the same bits with about half the arithmetic.
+constexpr int DIST_L2RGBD_CD = 4;
+static_assert(DIST_L2RGBD_CD != DIST_L2 && DIST_L2RGBD_CD != DIST_L1 && DIST_L2RGBD_CD != DIST_L2COMBINED &&
+                  DIST_L2RGBD_CD != DIST_L2RGBD,
+              "kernel variant id outside the distance kinds");
+
 template <int KIND>
 struct Feat {
-    static constexpr bool grad = (KIND == DIST_L2COMBINED || KIND == DIST_L2RGBD);
+    static constexpr bool grad = (KIND == DIST_L2COMBINED || KIND == DIST_L2RGBD || KIND == DIST_L2RGBD_CD);
 };
 
 // target patch accessors: f < C colour channel f, f >= C gradient component f - C
@@ -187,6 +197,20 @@ __device__ inline void fly_row(const float (&cur)[P + 1][C], const float (&nxt)[
             }
             dist += (double)(wt * norm);
             gdist += (double)(wt * gnorm);
+        } else if constexpr (KIND == DIST_L2RGBD_CD) {   // C == 4: colour values, depth gradient
+            double norm = 0.0, gnorm_d = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float d = cur[dx][c] - t.v(c, dx, dy);
+                norm += (double)(d * d);
+            }
+            {
+                float gx = g[6] - t.v(C + 6, dx, dy);
+                float gy = g[7] - t.v(C + 7, dx, dy);
+                gnorm_d += (double)(gx * gx + gy * gy);
+            }
+            dist += (double)wt * norm;
+            gdist_d += (double)wt * gnorm_d;
         } else {   // DIST_L2RGBD, C == 4
             double norm = 0.0, norm_d = 0.0, gnorm = 0.0, gnorm_d = 0.0;
 #pragma unroll
@@ -731,6 +755,7 @@ __device__ void run_pool(const PMArgs &a, const float *__restrict__ wl, const fl
     auto partial = [&]() {
         double v = lam * dist + lam1 * gdist;
         if constexpr (KIND == DIST_L2RGBD) v = v + lamd * dist_d + lamd1 * gdist_d;
+        if constexpr (KIND == DIST_L2RGBD_CD) v = dist + gdist_d;   // the weights 1, 0, 0, 1
         return v;
     };
     auto record = [&](float d) {
@@ -831,7 +856,7 @@ __device__ __attribute__((always_inline)) inline void sweep_pool_body(const PMAr
     constexpr int NT = kBlock * WPR;             // threads per block
     constexpr int kSeg = TX / WPR;               // targets of one tile row in a wave
     constexpr int kWaveTargets = TPB * kSeg;
-    static_assert(PS > 0 && (KIND == DIST_L2RGBD || KIND == DIST_L2 || KIND == DIST_L1 ||
+    static_assert(PS > 0 && (KIND == DIST_L2RGBD || KIND == DIST_L2RGBD_CD || KIND == DIST_L2 || KIND == DIST_L1 ||
                              (KIND == DIST_L2COMBINED && (C == 1 || C == 3))),
                   "pool sweep: compile-time patch; gradient distances need the on-the-fly source gradient");
     extern __shared__ __align__(16) float plds[];
@@ -1535,7 +1560,12 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     // (9C + 4) (gradient distances) or (3C + 2) (colour-only distances)
     int R = 0;
     for (int ws = a.maxw; ws >= 1; ws /= 2) R++;
-    const double per_pixel = (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) ? 9.0 * C + 4 : 3.0 * C + 2;
+    // L2RGBD at lambda_rgb 1, lambda_d 0 runs the variant without the zero-weighted terms:
+    // 19 flops a pixel (3 colour differences, squares and sums; 2 depth forward
+    // differences, 2 differences, squares and a sum; 2 weighted accumulations)
+    const bool rgbd_cd = d.kind == DIST_L2RGBD && d.lambda == 1.0f && d.lambda_d == 0.0f;
+    const double per_pixel =
+        rgbd_cd ? 19.0 : (d.kind == DIST_L2COMBINED || d.kind == DIST_L2RGBD) ? 9.0 * C + 4 : 3.0 * C + 2;
     const double sweep_flops = (double)pr.targets * (2 + R) * d.patch * d.patch * per_pixel;
     // candidate-pool sweep for the specialised gradient distances (up to kMaxR windows)
     const bool pool = pool_sweep_fits(sW, sH, R) && ((d.kind == DIST_L2COMBINED && C == 3 &&
@@ -1553,7 +1583,8 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     (arr == 2 ? k_pm_sweep_pool<KD, CC, PP, 2, 1>                                                     \
               : (arr == 3 ? k_pm_sweep_pool<KD, CC, PP, 1, 2> : k_pm_sweep_pool<KD, CC, PP, 1, 1>))     \
         <<<blocks, kBlock * wpr, lds_pool, s>>>(a)
-        if (d.kind == DIST_L2RGBD) NLI_POOL(DIST_L2RGBD, 4, 9);
+        if (rgbd_cd) NLI_POOL(DIST_L2RGBD_CD, 4, 9);
+        else if (d.kind == DIST_L2RGBD) NLI_POOL(DIST_L2RGBD, 4, 9);
         else if (d.kind == DIST_L2 && d.patch == 7) NLI_POOL(DIST_L2, 3, 7);
         else if (d.kind == DIST_L2) NLI_POOL(DIST_L2, 3, 9);
         else if (d.kind == DIST_L1 && d.patch == 7) NLI_POOL(DIST_L1, 3, 7);
@@ -1577,7 +1608,8 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
     int arr = pool ? pm_tiles_per_block() : 0;
     if (arr == 3 && a.row_scan) arr = 1;
     const uint64_t tune_key = ((uint64_t)d.kind << 56) | ((uint64_t)C << 48) | ((uint64_t)d.patch << 40) |
-                              ((uint64_t)(split_ok ? 1 : 0) << 34) | ((uint64_t)(a.row_scan ? 1 : 0) << 32) |
+                              ((uint64_t)(rgbd_cd ? 1 : 0) << 35) | ((uint64_t)(split_ok ? 1 : 0) << 34) |
+                              ((uint64_t)(a.row_scan ? 1 : 0) << 32) |
                               (uint64_t)(32 - __builtin_clz((unsigned)pr.ntiles | 1u));
     if (pool && arr == 0) arr = tuned_arrangement(tune_key);
     auto set_sweep = [&](int k) {
