@@ -1595,7 +1595,8 @@ void patchmatch(const DistDesc &d, const PMConfig &cfg, const float *src, const 
 #undef NLI_POOL
         NLI_LAUNCH_CHECK();
     };
-    // tile arrangement: forced, tuned earlier for this (kind, channels, patch, row scan,
+    // tile arrangement: forced, tuned earlier for this key (kind, the L2RGBD lambda (1, 0)
+    // variant, channels, patch, row scan, whether the split-row arrangement is offered,
     // size class), the default before tuning, or tuned now on the first sweep (every
     // arrangement gives identical fields).  The split-row arrangement 3 needs the row scan
     // off (forced 3 with the scan on runs arrangement 1), and it pays only where the tiles
