@@ -608,12 +608,11 @@ def test_process_psnr_1024_vs_reference_runs(ctx, nli, syn):
     parameters with FP64 CG): whole GPU runs against whole runs of the reference's own
     process() (serial build, srand(seed): tests/golden/make_process_psnr_1024.py, about 35
     minutes a run).  Per image the mean hole PSNR over 4 GPU seeds is compared with the
-    mean over the reference's seeds.  One run varies by 0.3-0.4 dB from seed to seed at
-    this size, so the reference runs that can be made leave a standard error near 0.1 dB
-    over the images: this size can show a deficit but cannot resolve the 0.1 dB bar in the
-    GPU's favour (that is resolved at the smaller sizes above).  The test fails on a
-    deficit the data resolves (the mean difference more than two standard errors below
-    -0.1 dB) and on a mean difference below -0.3 dB."""
+    mean over the reference's seeds (6 seeds an image, 47 runs).  One run varies by
+    0.3-0.4 dB from seed to seed at this size, which leaves a standard error near 0.06 dB
+    over the images: the bar is one-sided here (the GPU is not worse: the mean difference
+    must not lie more than two standard errors below -0.1 dB); it is resolved in both
+    directions at the smaller sizes above."""
     g = golden("process_psnr_1024")
     images, ref = g["images"], g["psnr"]   # psnr[image, seed]; nan where not run
     n_ref = np.sum(np.isfinite(ref), axis=1)
@@ -634,8 +633,7 @@ def test_process_psnr_1024_vs_reference_runs(ctx, nli, syn):
     se = d.std(ddof=1) / np.sqrt(len(d))
     print(f"1024x1024: GPU - reference hole PSNR per image {np.round(d, 3)}; mean {d.mean():+.3f} dB, "
           f"SE {se:.3f} dB over {len(d)} images ({int(n_ref.sum())} reference runs)")
-    assert d.mean() + 2 * se >= -0.1
-    assert d.mean() >= -0.3
+    assert d.mean() - 2 * se >= -0.1
 
 
 def test_process_psnr_rgb_pipeline_vs_reference_run(ctx, nli):
